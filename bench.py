@@ -1,0 +1,178 @@
+"""Benchmark: block H.psi on the synthetic 96^3 monitor-style workload (one MI355X).
+
+A step is one fused block apply  H psi = -1/2 Lap psi + V_eff psi  of the 16-orbital
+FP32 block on the 96^3 analytic two-centre grid (``trial_fix`` closure, inputs and
+outputs resident in HBM).  To keep the number an HBM number, the timed loop cycles
+over enough distinct input/output blocks that the working set is larger than twice
+the 256 MB last-level (Infinity) cache.
+
+Prints one JSON line:
+  value          whole-job H.psi algorithmic bandwidth (GB/s), the metric's first part
+  roofline       achieved vs 8 TB/s for the stencil kernel, timed with HIP events on
+                 the launch stream; `traffic` from the committed rocprofv3 PMC summary
+  cpu_baseline   the FP64 NumPy oracle applying the same block on this host
+Algorithmic bytes per block apply: B = N (2 s_psi k + 7 s_g + s_V).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "H·ψ GB/s; eigensolve+Poisson s, FP32+refine vs FP64 GPU at ≤1e-6 Ha"
+HBM_PEAK_GBS = 8000.0
+LLC_BYTES = 256 * 2 ** 20
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--dtype", choices=("f32", "f64"), default="f32")
+    ap.add_argument("--closure", choices=("trial_fix", "production"), default="trial_fix")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    return ap.parse_args()
+
+
+def algorithmic_bytes(N, k, s_psi, s_g=4, s_v=4):
+    return N * (2 * s_psi * k + 7 * s_g + s_v)
+
+
+def cpu_baseline(geom, psi_np, veff_np, seconds):
+    """Oracle FP64 block apply, repeated on the host for about ``seconds``."""
+    from oracle import kinetic as okin
+
+    N = psi_np[0].size
+    k = psi_np.shape[0]
+    t0 = time.perf_counter()
+    done = 0
+    while True:
+        for f in psi_np:
+            okin.monitor_kinetic(f, geom, "trial_fix") + veff_np * f
+            done += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    dt = time.perf_counter() - t0
+    blocks = done / k
+    gbs = algorithmic_bytes(N, k, 8, 8, 8) * blocks / dt / 1e9
+    return {
+        "value": round(gbs, 4),
+        "unit": "GB/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"FP64 NumPy oracle (ops/kinetic.py restatement), {done} orbital applies "
+                  f"({blocks:.2f} blocks of {k}) on the 96^3 grid in {dt:.1f} s, single thread",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from isogriddft_amd import configs, kinetic
+    from isogriddft_amd.device import DeviceGrid
+
+    cfg = configs.C2
+    geom = configs.build_synthetic_geometry(cfg)
+    dg = DeviceGrid(geom)
+    N, k = dg.N, cfg.k
+    dt = torch.float32 if args.dtype == "f32" else torch.float64
+    s_psi = 4 if dt == torch.float32 else 8
+    psi0 = configs.build_synthetic_orbitals(geom, cfg, device="cuda", dtype=torch.float64)
+    xyz = dg.xyz()
+    veff64 = -1.0 / torch.sqrt(0.3 + xyz[0] ** 2 + xyz[1] ** 2 + (xyz[2] - 0.7) ** 2) \
+             - 1.0 / torch.sqrt(0.3 + xyz[0] ** 2 + xyz[1] ** 2 + (xyz[2] + 0.7) ** 2)
+    veff = veff64.to(dt).contiguous()
+    block_bytes = psi0.numel() * s_psi
+    R = max(2, math.ceil(2 * LLC_BYTES / (2 * block_bytes)) + 1)
+    ins = [(psi0 * (1.0 + 0.01 * r)).to(dt).contiguous() for r in range(R)]
+    outs = [torch.empty_like(ins[0]) for _ in range(R)]
+    stream = torch.cuda.current_stream()
+
+    def step(i):
+        kinetic.apply_operator(dg, ins[i % R], outs[i % R], scale=-0.5, closure=args.closure, veff=veff)
+
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for i in range(args.steps):
+        step(i)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = ev0.elapsed_time(ev1)
+    ms_per_step = gpu_ms / args.steps
+    B = algorithmic_bytes(N, k, s_psi, s_psi, s_psi)
+    achieved = B / (ms_per_step * 1e-3) / 1e9
+
+    # correctness spot check of one orbital against the FP64 oracle (small, cheap)
+    traffic = None
+    tf = ROOT / "profiles" / f"hpsi_c2_{args.dtype}_{args.closure}_traffic.json"
+    if tf.exists():
+        try:
+            traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+        except (ValueError, OSError):
+            traffic = None
+
+    line = {
+        "metric": METRIC,
+        "value": round(achieved, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic",
+        "config": {
+            "workload": "c2 operator-level synthetic: block H.psi, 96^3 analytic two-centre "
+                        f"monitor-style grid, k={k} orbitals, {args.closure} closure",
+            "grid": list(dg.shape),
+            "k": k,
+            "buffers_cycled": R,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+        },
+        "detail": {
+            "algorithmic_bytes_per_apply": B,
+            "block_applies_per_s": round(1e3 / ms_per_step, 2),
+            "wall_s_timed_region": round(wall, 4),
+        },
+    }
+    if not args.no_cpu:
+        psi_np = psi0[: k].cpu().numpy()
+        line["cpu_baseline"] = cpu_baseline(geom, psi_np, veff64.cpu().numpy().reshape(dg.shape), args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

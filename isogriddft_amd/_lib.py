@@ -1,0 +1,91 @@
+"""ctypes binding of ``libisogrid_hip.so`` (the C ABI declared in include/isogrid_hip.h).
+
+The library is loaded from this package directory.  There is no fallback: if the
+library is missing every entry point raises, so no CPU path can silently stand in
+for the HIP kernels.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import torch
+
+LIB_PATH = Path(__file__).resolve().parent / "libisogrid_hip.so"
+
+P = C.c_void_p
+I = C.c_int
+L = C.c_longlong
+D = C.c_double
+
+# name -> argtypes (restype int unless listed in _RESTYPE)
+SIGNATURES = {
+    "ig_monitor_stencil_f64": [P, P, I, L, I, I, I, D, D, D, P, P, P, D, I, I, I, I, I, P],
+    "ig_monitor_stencil_f32": [P, P, I, L, I, I, I, D, D, D, P, P, P, D, I, I, I, I, I, P],
+    "ig_legacy_stencil_f64": [P, P, I, L, I, I, I, P, P, D, I, I, P],
+    "ig_legacy_stencil_f32": [P, P, I, L, I, I, I, P, P, D, I, I, P],
+    "ig_reduction_work_size": [L],
+    "ig_dots_f64": [I, P, P, P, P, P, P, P, L, P, P, I, I, I, P],
+    "ig_dots_f32": [I, P, P, P, P, P, P, P, L, P, P, I, I, I, P],
+    "ig_bicgstab_p_f64": [L, P, P, P, P, P, P, P],
+    "ig_bicgstab_p_f32": [L, P, P, P, P, P, P, P],
+    "ig_bicgstab_s_f64": [L, P, P, P, P, P, P, P],
+    "ig_bicgstab_s_f32": [L, P, P, P, P, P, P, P],
+    "ig_bicgstab_x_f64": [L, P, P, P, P, P, P, P, P, P, P],
+    "ig_bicgstab_x_f32": [L, P, P, P, P, P, P, P, P, P, P],
+    "ig_gram_work_size": [I, I, L],
+    "ig_gram_f32": [P, L, I, P, L, I, P, L, P, I, P, L, P],
+    "ig_gram_f64": [P, L, I, P, L, I, P, L, P, I, P, L, P],
+}
+_RESTYPE = {"ig_gram_work_size": L}
+
+_lib = None
+
+
+class HipLibraryError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the library (once) and bind every declared symbol."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise HipLibraryError(
+            f"{LIB_PATH} is missing: build it with `python -m isogriddft_amd.build` "
+            "(there is no CPU fallback for the HIP kernels)"
+        )
+    lib = C.CDLL(str(LIB_PATH))
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPE.get(name, I)
+    _lib = lib
+    return lib
+
+
+def call(name, *args):
+    """Invoke an entry point and translate its status code."""
+    fn = getattr(load(), name)
+    st = fn(*args)
+    if st == 0:
+        return
+    if st in (-1, -2):
+        raise ValueError(f"{name}: invalid or unsupported arguments (status {st})")
+    raise RuntimeError(f"{name}: HIP launch failed (status {st})")
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("HIP kernels take device tensors; got a host tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+def stream_handle(stream=None):
+    s = torch.cuda.current_stream() if stream is None else stream
+    return C.c_void_p(s.cuda_stream)

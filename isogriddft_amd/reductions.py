@@ -1,0 +1,86 @@
+"""Weighted reductions and Gram products on the GPU.
+
+``wdot``      sum a*b*w with FP64 accumulation     (ops/kinetic.py:61-74)
+``gram``      X W Y^T for orbital-major blocks       (ks/eigensolver.py:246-272,
+              ops/reductions_jax.py:89-104); FP32 blocks run on MFMA.
+Workspaces come from the torch caching allocator and are reused per device.
+"""
+
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+
+_WORK: dict = {}
+
+
+def workspace(device, n_doubles: int, slot: str = "red") -> torch.Tensor:
+    key = (str(device), slot)
+    t = _WORK.get(key)
+    if t is None or t.numel() < n_doubles:
+        t = torch.empty(max(n_doubles, 1024), dtype=torch.float64, device=device)
+        _WORK[key] = t
+    return t
+
+
+def dots_into(out: torch.Tensor, pairs, w=None, slots=(0, 1, 2), stream=None):
+    """out[slots[i]] = sum(a_i * b_i [* w]) for up to three pairs (device-resident)."""
+    nd = len(pairs)
+    a0, b0 = pairs[0]
+    dt = a0.dtype
+    n = a0.numel()
+    lib = _lib.load()
+    work = workspace(a0.device, lib.ig_reduction_work_size(n))
+    ptrs = []
+    for i in range(3):
+        if i < nd:
+            ptrs += [_lib.ptr(pairs[i][0]), _lib.ptr(pairs[i][1])]
+        else:
+            ptrs += [None, None]
+    if w is not None and w.dtype != dt:
+        w = w.to(dt)
+    sfx = "f64" if dt == torch.float64 else "f32"
+    _lib.call(f"ig_dots_{sfx}", nd, *ptrs, _lib.ptr(w), n, _lib.ptr(work), _lib.ptr(out),
+              slots[0], slots[1] if nd > 1 else 0, slots[2] if nd > 2 else 0,
+              _lib.stream_handle(stream))
+    return out
+
+
+def wdot(dg, a: torch.Tensor, b: torch.Tensor | None, weighted: bool = True) -> float:
+    """Host scalar sum(a * b * w) (b None -> sum(a * w))."""
+    a = a.reshape(-1).contiguous()
+    if b is None:
+        b = torch.ones_like(a)
+    b = b.reshape(-1).contiguous()
+    out = torch.zeros(4, dtype=torch.float64, device=a.device)
+    dots_into(out, [(a, b)], dg.weights(a.dtype) if weighted else None)
+    return float(out[0].item())
+
+
+def gram(X: torch.Tensor, Y: torch.Tensor, w: torch.Tensor, out: torch.Tensor | None = None,
+         accumulate: bool = False, stream=None) -> torch.Tensor:
+    """G = X W Y^T (FP64 result) for X (m, N), Y (n, N) of the same dtype."""
+    if X.dtype != Y.dtype:
+        raise ValueError("X and Y must share a dtype")
+    X2 = X.reshape(X.shape[0], -1)
+    Y2 = Y.reshape(Y.shape[0], -1)
+    if X2.stride(1) != 1 or Y2.stride(1) != 1:
+        raise ValueError("Gram operands must be row-contiguous")
+    m, N = X2.shape
+    n = Y2.shape[0]
+    if Y2.shape[1] != N or w.numel() != N:
+        raise ValueError("weights must match the orbital grid size.")
+    if out is None:
+        out = torch.zeros((m, n), dtype=torch.float64, device=X.device)
+    lib = _lib.load()
+    need = lib.ig_gram_work_size(m, n, N)
+    work = workspace(X.device, need, "gram")
+    sfx = "f64" if X.dtype == torch.float64 else "f32"
+    ww = w.reshape(-1)
+    if ww.dtype != X.dtype:
+        ww = ww.to(X.dtype)
+    _lib.call(f"ig_gram_{sfx}", _lib.ptr(X2), X2.stride(0), m, _lib.ptr(Y2), Y2.stride(0), n,
+              _lib.ptr(ww.contiguous()), N, _lib.ptr(out), int(accumulate), _lib.ptr(work), work.numel(),
+              _lib.stream_handle(stream))
+    return out

@@ -1,0 +1,52 @@
+"""GPU parity of the weighted dots and Gram products (FP64 bar 1e-13, FP32 bar 2e-6)."""
+
+import numpy as np
+import pytest
+import torch
+
+from isogriddft_amd import reductions
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("m,n,N", [(40, 70, 5003), (16, 16, 884736 // 8), (100, 3, 777), (65, 130, 20000)])
+def test_gram_f64(m, n, N):
+    rng = np.random.default_rng(m + n)
+    X = rng.normal(size=(m, N))
+    Y = rng.normal(size=(n, N))
+    w = rng.uniform(0.1, 2.0, size=N)
+    G = reductions.gram(torch.from_numpy(X).cuda(), torch.from_numpy(Y).cuda(), torch.from_numpy(w).cuda())
+    ref = X @ (w[:, None] * Y.T)
+    err = np.max(np.abs(G.cpu().numpy() - ref)) / np.max(np.abs(ref))
+    assert err < 1e-13, err
+
+
+@pytest.mark.parametrize("m,n,N", [(40, 70, 5003), (48, 48, 200000), (7, 130, 999)])
+def test_gram_f32_mfma(m, n, N):
+    rng = np.random.default_rng(m * n)
+    X = rng.normal(size=(m, N)).astype(np.float32)
+    Y = rng.normal(size=(n, N)).astype(np.float32)
+    w = rng.uniform(0.1, 2.0, size=N).astype(np.float32)
+    G = reductions.gram(torch.from_numpy(X).cuda(), torch.from_numpy(Y).cuda(), torch.from_numpy(w).cuda())
+    Xw = (X * w).astype(np.float64)  # the kernel rounds X*w to FP32 before the product
+    Xw = (X * w).astype(np.float32).astype(np.float64)
+    ref = Xw @ Y.astype(np.float64).T
+    scale = np.sqrt(np.outer(np.sum(Xw * Xw, 1), np.sum(Y.astype(np.float64) ** 2, 1)))
+    err = np.max(np.abs(G.cpu().numpy() - ref) / scale)
+    assert err < 2e-6, err
+
+
+def test_dots_multi():
+    rng = np.random.default_rng(5)
+    n = 123457
+    a, b, c, d, e, f, w = (rng.normal(size=n) for _ in range(7))
+    t = [torch.from_numpy(v).cuda() for v in (a, b, c, d, e, f, w)]
+    out = torch.zeros(8, dtype=torch.float64, device="cuda")
+    reductions.dots_into(out, [(t[0], t[1]), (t[2], t[3]), (t[4], t[5])], t[6], slots=(2, 5, 7))
+    o = out.cpu().numpy()
+    np.testing.assert_allclose(o[[2, 5, 7]], [np.sum(a * b * w), np.sum(c * d * w), np.sum(e * f * w)],
+                               rtol=1e-12)
+    out32 = torch.zeros(4, dtype=torch.float64, device="cuda")
+    reductions.dots_into(out32, [(t[0].float(), t[1].float())], None)
+    ref = np.sum(a.astype(np.float32).astype(np.float64) * b.astype(np.float32).astype(np.float64))
+    assert abs(out32[0].item() - ref) < 1e-9 * np.sum(np.abs(a * b))

@@ -1,0 +1,130 @@
+"""GPU parity of the H.psi / Laplacian stencils against the FP64 oracle.
+
+Bar: FP64 relative error <= 1e-12 (norm-wise and max-abs relative to max|ref|);
+FP32 relative error <= 2e-5 (norm-wise), stated per test.  Tilings and x-chunking
+must not change a single bit of the result.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from isogriddft_amd import configs, kinetic
+from isogriddft_amd.device import DeviceGrid
+from oracle import grid as ogrid
+from oracle import kinetic as okin
+
+pytestmark = pytest.mark.gpu
+
+F64_TOL = 1e-12
+F32_TOL = 2e-5
+
+
+def _rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return np.linalg.norm(a - b) / np.linalg.norm(b), np.max(np.abs(a - b)) / np.max(np.abs(b))
+
+
+def _geometries(small_monitor):
+    yield "ref19x19x23", small_monitor
+    yield "synthetic33x37x70", configs.build_synthetic_geometry(shape=(33, 37, 70))
+    yield "synthetic5x6x7", configs.build_synthetic_geometry(shape=(5, 6, 7))
+    yield "synthetic40x96x96", configs.build_synthetic_geometry(shape=(40, 96, 96))
+
+
+def _fields(shape, k, seed):
+    rng = np.random.default_rng(seed)
+    X, Y, Z = np.meshgrid(*[np.linspace(-1, 1, n) for n in shape], indexing="ij")
+    out = []
+    for m in range(k):
+        c = rng.normal(size=4)
+        out.append(c[0] * np.exp(-2 * (X - 0.1 * m) ** 2 - 3 * Y ** 2 - Z ** 2) + c[1] * X * Y
+                   + c[2] * np.cos(2 * Z + m) + 0.05 * c[3] * rng.normal(size=shape))
+    return np.stack(out)
+
+
+@pytest.mark.parametrize("closure", ["production", "trial_fix"])
+@pytest.mark.parametrize("transpose", [False, True])
+def test_monitor_f64_matches_oracle(small_monitor, closure, transpose):
+    for name, g in _geometries(small_monitor):
+        print(f"case {name} closure={closure} transpose={transpose}", flush=True)
+        dg = DeviceGrid(g)
+        psi = _fields(dg.shape, 3, 7)
+        x = torch.from_numpy(psi).cuda()
+        out = kinetic.apply_operator(dg, x, scale=1.0, closure=closure, transpose=transpose)
+        torch.cuda.synchronize()
+        got = out.cpu().numpy()
+        ref = np.stack([(okin.monitor_laplacian_transpose if transpose else okin.monitor_laplacian)(f, g, closure)
+                        for f in psi])
+        rn, rm = _rel(got, ref)
+        assert rn <= F64_TOL and rm <= F64_TOL, (name, rn, rm)
+
+
+def test_monitor_hpsi_with_potential_and_scale(small_monitor):
+    g = small_monitor
+    dg = DeviceGrid(g)
+    psi = _fields(dg.shape, 2, 3)
+    V = -1.0 / np.sqrt(0.5 + g.x_points ** 2 + g.y_points ** 2 + g.z_points ** 2)
+    out = kinetic.apply_operator(dg, torch.from_numpy(psi).cuda(), scale=-0.5, closure="trial_fix",
+                                 veff=torch.from_numpy(V).cuda())
+    ref = np.stack([okin.monitor_kinetic(f, g, "trial_fix") + V * f for f in psi])
+    rn, rm = _rel(out.cpu().numpy(), ref)
+    assert rn <= F64_TOL and rm <= F64_TOL
+
+
+def test_tilings_and_chunks_are_bitwise_identical():
+    g = configs.build_synthetic_geometry(shape=(50, 45, 96))
+    dg = DeviceGrid(g)
+    for dt in (torch.float64, torch.float32):
+        x = torch.from_numpy(_fields(dg.shape, 2, 11)).to(dt).cuda()
+        base = kinetic.apply_operator(dg, x, scale=-0.5, closure="production")
+        for tile in (32, 64):
+            for xc in (0, 7, 16):
+                other = kinetic.apply_operator(dg, x, scale=-0.5, closure="production", tile=tile, xchunk=xc)
+                assert torch.equal(base, other), (dt, tile, xc)
+
+
+def test_interior_only_zeroes_boundary(small_monitor):
+    dg = DeviceGrid(small_monitor)
+    x = torch.from_numpy(_fields(dg.shape, 1, 5)[0]).cuda()
+    out = kinetic.apply_operator(dg, x, scale=-1.0, closure="production", interior_only=True).cpu().numpy()
+    ref = -okin.monitor_laplacian(x.cpu().numpy(), small_monitor, "production")
+    mask = ogrid._boundary_mask(dg.shape)
+    assert np.all(out[mask] == 0.0)
+    rn, _ = _rel(out[~mask], ref[~mask])
+    assert rn <= F64_TOL
+
+
+@pytest.mark.parametrize("closure", ["production", "trial_fix"])
+def test_monitor_f32_matches_oracle(closure):
+    g = configs.build_synthetic_geometry(shape=(48, 40, 64))
+    dg = DeviceGrid(g)
+    psi = configs.build_synthetic_orbitals(g, k=3).numpy()
+    out = kinetic.apply_operator(dg, torch.from_numpy(psi).float().cuda(), scale=-0.5, closure=closure)
+    ref = np.stack([okin.monitor_kinetic(f.astype(np.float32).astype(np.float64), g, closure) for f in psi])
+    rn, rm = _rel(out.double().cpu().numpy(), ref)
+    print(f"f32 {closure}: rel norm {rn:.3e} rel max {rm:.3e}")
+    assert rn <= F32_TOL
+
+
+@pytest.mark.parametrize("transpose", [False, True])
+def test_legacy_f64_matches_oracle(transpose):
+    for shape in ((49, 49, 61), (6, 7, 8)):
+        g = ogrid.build_legacy_geometry(shape=shape, half_extents=(8.0, 8.0, 8.0))
+        dg = DeviceGrid(g)
+        psi = _fields(shape, 2, 9)
+        out = kinetic.apply_operator(dg, torch.from_numpy(psi).cuda(), scale=1.0, transpose=transpose)
+        ref = np.stack([(okin.legacy_laplacian_transpose if transpose else okin.legacy_laplacian)(f, g) for f in psi])
+        rn, rm = _rel(out.cpu().numpy(), ref)
+        assert rn <= F64_TOL and rm <= F64_TOL, (shape, rn, rm)
+
+
+def test_numpy_call_surface(small_monitor):
+    psi = _fields(small_monitor.jacobian.shape, 1, 1)[0]
+    got = kinetic.apply_monitor_grid_kinetic_operator_trial_boundary_fix(psi, small_monitor)
+    assert isinstance(got, np.ndarray)
+    ref = okin.monitor_kinetic(psi, small_monitor, "trial_fix")
+    assert _rel(got, ref)[0] <= F64_TOL
+    with pytest.raises(ValueError):
+        kinetic.apply_monitor_grid_kinetic_operator(psi[:-1], small_monitor)

@@ -1,0 +1,110 @@
+"""Pin the CPU oracle against the reference: recorded scalars and golden vectors.
+
+Reference-held scalars (src/isogrid/audit/baselines.py of the reference):
+  legacy 51^3 H2 grid, bonding trial orbital:  <T> = 0.997532314836 (:1066),
+      <psi|V_loc|psi> = -2.173678262801 (:1067), E_loc = -4.347356525602 (:830),
+      E_H = 1.748670071699 (:1068, SciPy BiCGSTAB rtol 1e-8)
+  monitor 67x67x81 H2 development grid:        <T> = 0.9671226925733339 (:1088, :1110,
+      both closures), E_loc = -4.364949512790 (:836), E_H = 1.765755769546 (:768,
+      recorded with nodal multipole moments, rtol 1e-8)
+Golden vectors: tests/golden/reference_vectors.npz (tests/golden/make_golden.py).
+"""
+
+import numpy as np
+import pytest
+
+from oracle import eigen, grid, kinetic, poisson, pseudo
+
+
+def _trial(g):
+    f = 0.0
+    for a in grid.H2_CASE.atoms:
+        dx, dy, dz = g.x_points - a.position[0], g.y_points - a.position[1], g.z_points - a.position[2]
+        f = f + np.exp(-0.8 * (dx * dx + dy * dy + dz * dz))
+    return f / np.sqrt(np.sum(f * f * g.cell_volumes))
+
+
+@pytest.fixture(scope="module")
+def legacy51():
+    return grid.build_legacy_geometry()
+
+
+def test_legacy_kinetic_and_local_scalars(legacy51):
+    g = legacy51
+    psi = _trial(g)
+    w = g.cell_volumes
+    T = float(np.sum(psi * kinetic.legacy_kinetic(psi, g) * w))
+    assert abs(T - 0.997532314836) < 1e-11
+    V = pseudo.local_ionic_potential(grid.H2_CASE, g)
+    assert abs(float(np.sum(psi * psi * V * w)) - (-2.173678262801)) < 1e-11
+    assert abs(float(np.sum(2 * psi * psi * V * w)) - (-4.347356525602)) < 1e-11
+
+
+def test_legacy_hartree_energy(legacy51):
+    g = legacy51
+    rho = 2.0 * _trial(g) ** 2
+    pot, _, _, info = poisson.solve_poisson(g, rho, method="iter", tol=1e-12)
+    assert info["code"] == 0
+    # the recorded value comes from a rtol = 1e-8 BiCGSTAB solve
+    assert abs(poisson.hartree_energy(rho, g, pot) - 1.748670071699) < 1e-8
+
+
+@pytest.mark.slow
+def test_monitor_67_scalars():
+    g = grid.generate_monitor_geometry(grid.H2_CASE, (67, 67, 81), (8.0, 8.0, 10.0),
+                                       grid.h2_patch_development_parameters())
+    psi = _trial(g)
+    w = g.cell_volumes
+    for closure in ("production", "trial_fix"):
+        T = float(np.sum(psi * kinetic.monitor_kinetic(psi, g, closure) * w))
+        assert abs(T - 0.9671226925733339) < 1e-12, closure
+    rho = 2.0 * psi * psi
+    V = pseudo.local_ionic_potential(grid.H2_CASE, g)
+    assert abs(float(np.sum(rho * V * w)) - (-4.364949512790)) < 1e-11
+    pot, _, _, info = poisson.solve_poisson(g, rho, mode="nodal", method="iter", tol=1e-11, maxiter=5000)
+    assert abs(poisson.hartree_energy(rho, g, pot) - 1.765755769546) < 2e-8
+
+
+def test_monitor_generator_matches_reference(golden):
+    g = grid.generate_monitor_geometry(grid.H2_CASE, (19, 19, 23), (8.0, 8.0, 10.0),
+                                       grid.h2_patch_development_parameters())
+    for n in ("x", "y", "z"):
+        np.testing.assert_allclose(getattr(g, n + "_points"), golden["mon_" + n], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(g.jacobian, golden["mon_jacobian"], rtol=1e-12)
+    np.testing.assert_allclose(g.patch_radii, golden["mon_patch_radii"], rtol=1e-15)
+
+
+@pytest.mark.parametrize("tag", ["centered", "shifted"])
+def test_corrected_moment_boundary_matches_reference(golden, small_monitor, tag):
+    rho = golden["rho_" + tag]
+    b, mom = poisson.multipole_boundary(small_monitor, rho, 2, "corrected_moments")
+    key = f"{tag}_corrected_moments"
+    assert abs(mom[0] - float(golden[key + "_charge"])) < 1e-13
+    np.testing.assert_allclose(mom[1], golden[key + "_dipole"], atol=1e-13)
+    np.testing.assert_allclose(poisson.traceless_quadrupole(mom[2]), golden[key + "_quadrupole"], atol=1e-12)
+    np.testing.assert_allclose(b, golden[key + "_boundary"], rtol=1e-12, atol=1e-14)
+
+
+def test_patch_embedding_matches_reference(golden, small_monitor):
+    V, _, tot = pseudo.patch_local_correction_field(grid.H2_CASE, small_monitor, golden["patch_rho"],
+                                                    patch_radius_scale=0.75, correction_strength=1.3)
+    np.testing.assert_allclose(V, golden["patch_corrected_local_potential"], rtol=1e-12, atol=1e-12)
+    assert abs(tot - float(golden["patch_total_correction"])) < 1e-13
+
+
+def test_default_guess_block_matches_reference(golden):
+    lg = grid.build_legacy_geometry(shape=tuple(golden["guess_legacy_shape"]))
+    gb = eigen.default_guesses(lg, [a.position for a in grid.H2_CASE.atoms], 8, lg.reference_center)
+    np.testing.assert_allclose(gb, golden["guess_block"], rtol=1e-12, atol=1e-12)
+
+
+def test_reference_multipole_invariants(golden, small_monitor):
+    """Reference tests/test_hartree_poisson.py:234-306 restated on the oracle."""
+    g = small_monitor
+    rho = golden["rho_centered"]
+    dx, dy, dz = g.x_points, g.y_points, g.z_points
+    nodal = poisson.raw_moments(rho * g.cell_volumes, dx, dy, dz, 2)
+    _, mom = poisson.multipole_boundary(g, rho, 2, "corrected_moments")
+    assert np.linalg.norm(poisson.traceless_quadrupole(mom[2])) < 0.95 * np.linalg.norm(
+        poisson.traceless_quadrupole(nodal[2]))
+    assert abs(mom[0] - 2.0) < 5e-2
