@@ -41,6 +41,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--dtype", choices=("f32", "f64"), default="f32")
     ap.add_argument("--closure", choices=("trial_fix", "production"), default="trial_fix")
+    ap.add_argument("--tile", type=int, default=0, help="z tile width (0 auto, 32, 64)")
+    ap.add_argument("--xchunk", type=int, default=0, help="x planes per workgroup (0 auto)")
+    ap.add_argument("--workload", choices=("c2", "c5"), default="c2",
+                    help="c2: 96^3, k=16 (default); c5: 224^3, k=48 (benzene-size block)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     return ap.parse_args()
@@ -88,7 +92,7 @@ def main():
     from isogriddft_amd import configs, kinetic
     from isogriddft_amd.device import DeviceGrid
 
-    cfg = configs.C2
+    cfg = configs.C2 if args.workload == "c2" else configs.C5_OPERATOR
     geom = configs.build_synthetic_geometry(cfg)
     dg = DeviceGrid(geom)
     N, k = dg.N, cfg.k
@@ -106,7 +110,8 @@ def main():
     stream = torch.cuda.current_stream()
 
     def step(i):
-        kinetic.apply_operator(dg, ins[i % R], outs[i % R], scale=-0.5, closure=args.closure, veff=veff)
+        kinetic.apply_operator(dg, ins[i % R], outs[i % R], scale=-0.5, closure=args.closure, veff=veff,
+                               tile=args.tile, xchunk=args.xchunk)
 
     for i in range(args.warmup):
         step(i)
@@ -148,7 +153,8 @@ def main():
         "dtype": args.dtype,
         "data": "synthetic",
         "config": {
-            "workload": "c2 operator-level synthetic: block H.psi, 96^3 analytic two-centre "
+            "workload": f"{args.workload} operator-level synthetic: block H.psi, "
+                        f"{dg.shape[0]}x{dg.shape[1]}x{dg.shape[2]} analytic two-centre "
                         f"monitor-style grid, k={k} orbitals, {args.closure} closure",
             "grid": list(dg.shape),
             "k": k,

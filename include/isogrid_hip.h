@@ -27,12 +27,14 @@ extern "C" {
 
 /* Monitor-grid curvilinear operator on a block of nblk fields:
  *   out = scale * Lap(in) + veff * in,   Lap = J^-1 sum_a D_a c^{ab} D_b,
- * c^{ab} = J g^{ab} given as coef[6*N] = (cxx, cyy, czz, cxy, cxz, cyz), invj[N] = 1/J,
- * h* the uniform logical spacings, closure 0 = zero-ghost ("trial_fix"),
- * 1 = one-sided edge_order=2 ("production"); transpose=1 applies the exact
- * Euclidean transpose Lap^T (then veff multiplies the untransformed input);
- * interior_only=1 writes zero on boundary nodes; xchunk = planes per workgroup
- * along x (0 = all); tile = 0 (auto), 32 or 64 (z tile width).  veff may be NULL.
+ * c^{ab} = J g^{ab}, D the centred first difference (1/2h) on the uniform logical axes.
+ * coef[6*N] holds the pre-scaled coefficients c^{ab} / (4 h_a h_b) in the order
+ * (xx, yy, zz, xy, xz, yz); invj[N] = 1/J.  closure 0 = zero-ghost ("trial_fix"),
+ * 1 = one-sided edge_order=2 ("production"); transpose=1 applies the exact Euclidean
+ * transpose Lap^T (then veff multiplies the untransformed input); interior_only=1
+ * writes zero on boundary nodes; xchunk = planes per workgroup along x (0 =
+ * automatic, >= nx = no split); tile = 0 (automatic), 32 / 64 (z tile width) or
+ * NB*10000 + TZ*100 + TY (NB orbitals per workgroup, TY x TZ tile).  veff may be NULL.
  * Replaces: ops/kinetic.py:212-358 (compute_monitor_grid_contravariant_flux_components,
  * apply_monitor_grid_laplacian_operator[_trial_boundary_fix],
  * apply_monitor_grid_kinetic_operator[_trial_boundary_fix]) with scale = -1/2,
@@ -41,15 +43,13 @@ extern "C" {
  * poisson/poisson_jax.py:140-178 / open_boundary.py:1032-1038 (scale = -1,
  * interior_only = 1). */
 int ig_monitor_stencil_f64(const double* in, double* out, int nblk, long long ld, int nx, int ny,
-                           int nz, double hx, double hy, double hz, const double* coef,
-                           const double* invj, const double* veff, double scale, int closure,
-                           int transpose, int interior_only, int xchunk, int tile,
-                           hipStream_t stream);
+                           int nz, const double* coef, const double* invj, const double* veff,
+                           double scale, int closure, int transpose, int interior_only,
+                           int xchunk, int tile, hipStream_t stream);
 int ig_monitor_stencil_f32(const float* in, float* out, int nblk, long long ld, int nx, int ny,
-                           int nz, double hx, double hy, double hz, const float* coef,
-                           const float* invj, const float* veff, double scale, int closure,
-                           int transpose, int interior_only, int xchunk, int tile,
-                           hipStream_t stream);
+                           int nz, const float* coef, const float* invj, const float* veff,
+                           double scale, int closure, int transpose, int interior_only,
+                           int xchunk, int tile, hipStream_t stream);
 
 /* Legacy separable 7-point operator:
  *   out = scale * sum_axes [a+ (f[i+1]-f[i]) - a- (f[i]-f[i-1])] + veff * in,
@@ -115,6 +115,93 @@ int ig_gram_f32(const float* X, long long ldx, int m, const float* Y, long long 
 int ig_gram_f64(const double* X, long long ldx, int m, const double* Y, long long ldy, int n,
                 const double* w, long long N, double* G, int accumulate, double* work,
                 long long work_elems, hipStream_t stream);
+
+
+/* ------------------------------------------------------------------ physics fields */
+
+/* Doubles of `work` needed by the reductions below (grid-size independent). */
+int ig_physics_work_size(long long N);
+
+/* Local GTH potential of natoms atoms on the grid points xyz[3*N] (SoA x|y|z):
+ * atoms = natoms x 9 doubles (x, y, z, Z_ion, r_loc, C1, C2, C3, C4), device memory.
+ * out (=|+=) sum_A [-Z erf(r / (sqrt2 r_loc)) / r + exp(-r^2/2r_loc^2) sum_i C_i (r/r_loc)^(2i-2)].
+ * Replaces: pseudo/local.py:138-180 (_screened_coulomb_term, _local_polynomial_term),
+ * :303-357 (evaluate_monitor_grid/legacy_local_ionic_potential). */
+int ig_vloc_gth_f64(const double* xyz, long long N, int natoms, const double* atoms, double* out,
+                    int accumulate, hipStream_t stream);
+/* out[3*natoms] = dE/dR_A of E = sum_p q_p V_loc(|r_p - R_A|) (q = rho * w), natoms <= 64.
+ * No reference counterpart (the reference has no forces); used by the adjoint path. */
+int ig_vloc_gth_grad_f64(const double* xyz, const double* q, long long N, int natoms,
+                         const double* atoms, double* work, double* out, hipStream_t stream);
+
+/* Spin-polarised Slater exchange + PW92 correlation per point: eps_xc (per electron),
+ * v_up, v_down; *flag |= 1 if a density is below -1e-14 or not finite (the reference
+ * raises ValueError there, xc/lsda.py:57-67).
+ * Replaces: xc/lsda.py:70-129 (evaluate_lsda_terms; the reference's functional is
+ * libxc 'lda,vwn', this kernel evaluates PW92). */
+int ig_lsda_pw92_f64(const double* rho_up, const double* rho_down, long long N, double* eps,
+                     double* v_up, double* v_down, int* flag, hipStream_t stream);
+
+/* ------------------------------------------------------------------ nonlocal projectors */
+
+/* sizeof of the projector descriptor {int l, m, i; double r_l, prefactor}. */
+int ig_projector_desc_size(void);
+/* P[nproj][box] = p_i^l(r) Y_lm(r) on the support box (deriv = 0) or its derivative with
+ * respect to the atom position component deriv-1 (deriv = 1..3).  box (host pointer) =
+ * {i0, j0, k0, ni, nj, nk}; atom_pos (host) = 3 doubles; descs (device) = nproj
+ * descriptors.  Replaces: pseudo/nonlocal.py:99-206 (real harmonics, radial projector,
+ * evaluate_atomic_projector_field) restricted to the support box. */
+int ig_projector_fields_f64(const double* xyz, int nx, int ny, int nz, const int* box,
+                            const double* atom_pos, int nproj, const void* descs, double* P,
+                            int deriv, hipStream_t stream);
+/* Doubles of `work` needed by ig_nonlocal_project_*. */
+int ig_nonlocal_work_size(int nblk, int nproj);
+/* coef[m][p] = <p | psi_m>_W over the box; d[m][p] = sum_q H[p][q] coef[m][q] (coef may be
+ * NULL).  box is a host pointer.  Replaces: pseudo/nonlocal.py:254-266 (projection
+ * coefficients and h-matrix resolution). */
+int ig_nonlocal_project_f64(const double* psi, long long ld, int nblk, int nx, int ny, int nz,
+                            const int* box, const double* P, int nproj, const double* w,
+                            const double* H, double* work, double* coef, double* d,
+                            hipStream_t stream);
+int ig_nonlocal_project_f32(const float* psi, long long ld, int nblk, int nx, int ny, int nz,
+                            const int* box, const double* P, int nproj, const double* w,
+                            const double* H, double* work, double* coef, double* d,
+                            hipStream_t stream);
+/* out_m += sum_p d[m][p] P[p] on the box (host pointer box).
+ * Replaces: pseudo/nonlocal.py:267-273 (atom_action accumulation). */
+int ig_nonlocal_apply_f64(double* out, long long ld, int nblk, int nx, int ny, int nz,
+                          const int* box, const double* P, int nproj, const double* d,
+                          hipStream_t stream);
+int ig_nonlocal_apply_f32(float* out, long long ld, int nblk, int nx, int ny, int nz,
+                          const int* box, const double* P, int nproj, const double* d,
+                          hipStream_t stream);
+
+/* ------------------------------------------------------------------ multipoles */
+
+/* Raw Cartesian moments sum_p rho w [frac] x^a y^b z^c of degree <= order (<= 4) about
+ * center (host pointer, 3 doubles); out = nmono(order) doubles in the order
+ * 1; x y z; xx xy xz yy yz zz; xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz; xxxx ... zzzz.
+ * frac may be NULL.  Replaces: poisson/open_boundary.py:223-264, :267-296, :816-861. */
+int ig_multipole_moments_f64(const double* rho, const double* w, const double* frac,
+                             const double* xyz, long long N, const double* center, int order,
+                             double* work, double* out, hipStream_t stream);
+/* out = boundary potential sum_l Q_l(r)/r^(2l+1) on the six faces, 0 inside; qcoef
+ * (device) holds the homogeneous polynomial coefficients of degree 0..order over the same
+ * monomial order.  Replaces: poisson/open_boundary.py:467-495, :888-986. */
+int ig_boundary_values_f64(const double* xyz, int nx, int ny, int nz, const double* center,
+                           int order, const double* qcoef, double* out, hipStream_t stream);
+/* Per selected cell (cells = ncells x (i, j, k), device int32): quadratic least-squares fit
+ * of rho on the 4x4x4 node stencil -> coef[13] per cell.
+ * Replaces: poisson/open_boundary.py:299-342 (_cell_and_neighbor_slices,
+ * _local_quadratic_fit_coefficients). */
+int ig_cell_fit_f64(const double* rho, const double* xyz, int nx, int ny, int nz, const int* cells,
+                    int ncells, double* coef, hipStream_t stream);
+/* Raw moments (<= order) of the fitted density sampled at the 2x2x2 sub-cell midpoints,
+ * weight rho_fit * J_trilinear * subvol.  Replaces: poisson/open_boundary.py:396-458. */
+int ig_cell_moments_f64(const double* xyz, const double* jac, int nx, int ny, int nz,
+                        const int* cells, int ncells, const double* coef, double subvol,
+                        const double* center, int order, double* work, double* out,
+                        hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -21,8 +21,8 @@ D = C.c_double
 
 # name -> argtypes (restype int unless listed in _RESTYPE)
 SIGNATURES = {
-    "ig_monitor_stencil_f64": [P, P, I, L, I, I, I, D, D, D, P, P, P, D, I, I, I, I, I, P],
-    "ig_monitor_stencil_f32": [P, P, I, L, I, I, I, D, D, D, P, P, P, D, I, I, I, I, I, P],
+    "ig_monitor_stencil_f64": [P, P, I, L, I, I, I, P, P, P, D, I, I, I, I, I, P],
+    "ig_monitor_stencil_f32": [P, P, I, L, I, I, I, P, P, P, D, I, I, I, I, I, P],
     "ig_legacy_stencil_f64": [P, P, I, L, I, I, I, P, P, D, I, I, P],
     "ig_legacy_stencil_f32": [P, P, I, L, I, I, I, P, P, D, I, I, P],
     "ig_reduction_work_size": [L],
@@ -37,6 +37,21 @@ SIGNATURES = {
     "ig_gram_work_size": [I, I, L],
     "ig_gram_f32": [P, L, I, P, L, I, P, L, P, I, P, L, P],
     "ig_gram_f64": [P, L, I, P, L, I, P, L, P, I, P, L, P],
+    "ig_physics_work_size": [L],
+    "ig_vloc_gth_f64": [P, L, I, P, P, I, P],
+    "ig_vloc_gth_grad_f64": [P, P, L, I, P, P, P, P],
+    "ig_lsda_pw92_f64": [P, P, L, P, P, P, P, P],
+    "ig_projector_desc_size": [],
+    "ig_projector_fields_f64": [P, I, I, I, P, P, I, P, P, I, P],
+    "ig_nonlocal_work_size": [I, I],
+    "ig_nonlocal_project_f64": [P, L, I, I, I, I, P, P, I, P, P, P, P, P, P],
+    "ig_nonlocal_project_f32": [P, L, I, I, I, I, P, P, I, P, P, P, P, P, P],
+    "ig_nonlocal_apply_f64": [P, L, I, I, I, I, P, P, I, P, P],
+    "ig_nonlocal_apply_f32": [P, L, I, I, I, I, P, P, I, P, P],
+    "ig_multipole_moments_f64": [P, P, P, P, L, P, I, P, P, P],
+    "ig_boundary_values_f64": [P, I, I, I, P, I, P, P, P],
+    "ig_cell_fit_f64": [P, P, I, I, I, P, I, P, P],
+    "ig_cell_moments_f64": [P, P, I, I, I, P, I, P, D, P, I, P, P, P],
 }
 _RESTYPE = {"ig_gram_work_size": L}
 

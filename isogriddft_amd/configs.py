@@ -82,6 +82,9 @@ class SyntheticConfig:
 
 
 C2 = SyntheticConfig()
+# The same analytic map and orbital recipe at the benzene-size block (224^3, k = 48,
+# box +-14 Bohr): the operator-level stand-in for the largest configuration.
+C5_OPERATOR = SyntheticConfig(shape=(224, 224, 224), half_extents=(14.0, 14.0, 14.0), k=48)
 
 
 def synthetic_amplitudes(cfg: SyntheticConfig = C2):

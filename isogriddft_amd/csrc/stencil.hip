@@ -6,25 +6,36 @@
 // ghost ("trial_fix", CL = 0) or NumPy's edge_order=2 one-sided ends ("production",
 // CL = 1).  TR = 1 gives the exact Euclidean transpose
 //     Lap^T x = sum_b D_b^T ( sum_a c^{ab} D_a^T (J^-1 x) ).
-// One fused pass: a workgroup owns a TY x TZ tile of the (y, z) plane and marches
+// The flux coefficients arrive pre-scaled, c^{ab} / (4 h_a h_b), so the differences D
+// are applied un-scaled.  One fused pass: a workgroup owns a TY x TZ tile of the (y, z)
+// plane and NB orbitals, and marches
 // along x; a six-plane LDS ring holds the input planes with a y/z halo, fluxes of
 // the current plane are staged in LDS, the x fluxes stay in registers.  The output
 // is   out = scale * Lap psi + V * psi   (V optional), optionally zeroed on the
 // boundary nodes (the Poisson interior restriction).
 #include "common.h"
 
+// No implicit FMA contraction in this file: every fused multiply-add is written
+// explicitly, so the flux of a point is bit-identical whether it is evaluated as a
+// tile point or as a halo point of the neighbouring (possibly overlapping) tile.
+#pragma clang fp contract(off)
+
 namespace {
+
+template <typename T>
+__device__ __forceinline__ T flux3(T c0, T g0, T c1, T g1, T c2, T g2) {
+  return fma(c0, g0, fma(c1, g1, c2 * g2));
+}
 
 template <typename T>
 struct MonParams {
   const T* in;
   T* out;
-  const T* coef;  // 6*N: cxx cyy czz cxy cxz cyz
+  const T* coef;  // 6*N: c^{ab} / (4 h_a h_b) for ab = xx yy zz xy xz yz
   const T* invj;  // N
   const T* veff;  // N or null
   long long ld;   // elements between consecutive orbitals (in and out)
   int nblk, nx, ny, nz;
-  T ih2x, ih2y, ih2z;  // 1/(2h)
   T scale;
   int interior_only;
   int nty, ntz, nxc, xc, units;
@@ -45,8 +56,8 @@ __device__ __forceinline__ int rowcoef(int row, int off, int n) {
 template <int CL, typename T, typename F>
 __device__ __forceinline__ T dfwd(int i, int n, F f) {
   if (CL == 0 || (i > 0 && i < n - 1)) return f(1) - f(-1);
-  if (i == 0) return T(-3) * f(0) + T(4) * f(1) - f(2);
-  return T(3) * f(0) - T(4) * f(-1) + f(-2);
+  if (i == 0) return fma(T(-3), f(0), fma(T(4), f(1), -f(2)));
+  return fma(T(3), f(0), fma(T(-4), f(-1), f(-2)));
 }
 
 // Transposed derivative row i (un-scaled): sum_d D[i+d][i] f(d).
@@ -59,7 +70,7 @@ __device__ __forceinline__ T dtr(int i, int n, F f) {
     const int row = i + d;
     if (row < 0 || row >= n) continue;
     const int c = rowcoef<CL>(row, -d, n);
-    if (c != 0) s += T(c) * f(d);
+    if (c != 0) s = fma(T(c), f(d), s);
   }
   return s;
 }
@@ -70,254 +81,384 @@ __device__ __forceinline__ T dop(int i, int n, F f) {
   return dfwd<CL, T>(i, n, f);
 }
 
-template <typename T, int TY, int TZ, int CL, bool TR>
+// Buffer-resource access (MUBUF, raw, stride 0): the hardware range check returns 0
+// for loads and drops stores at byte offsets >= num_records.  Every field here is
+// smaller than BAD bytes, so BAD (or BAD + a valid offset) marks an inactive access:
+// zero ghosts, inactive tile points and suppressed stores need no branch.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+constexpr unsigned BAD = 0x40000000u;
+
+template <typename T>
+__device__ __forceinline__ rsrc_t mkbuf(const T* ptr, long long n) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(ptr), (short)0, (int)(n * (long long)sizeof(T)),
+                                           0x00020000);
+}
+__device__ __forceinline__ float bld(rsrc_t r, unsigned off, float) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ double bld(rsrc_t r, unsigned off, double) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ void bst(rsrc_t r, unsigned off, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)off, 0, 0);
+}
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void bst(rsrc_t r, unsigned off, double v) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)off, 0, 0);
+}
+
+template <typename T, int TY, int TZ, int CL, bool TR, int NB>
 __global__ __launch_bounds__(IG_THREADS) void monitor_stencil_kernel(MonParams<T> p) {
   constexpr int HF = TR ? 2 : 1;  // flux halo
   constexpr int HP = 2 * HF;      // input halo
   constexpr int PY = TY + 2 * HP;
   constexpr int PZ = TZ + 2 * HP;
   constexpr int PLANE = PY * PZ;
-  constexpr int NSLOT = 6;
+  constexpr int NSLOT = 6;             // ring of input planes; also the unroll period
   constexpr int RG = IG_THREADS / TZ;  // row groups
   constexpr int RPT = TY / RG;         // rows per thread
+  constexpr int NLD = (PLANE + IG_THREADS - 1) / IG_THREADS;  // ring elements per thread
+  constexpr int NFULL = PLANE / IG_THREADS;
+  constexpr int NHY = 2 * HF * TZ;     // y-halo flux points
+  constexpr int NHZ = 2 * HF * TY;     // z-halo flux points
+  constexpr int NHI = (NHY + NHZ + IG_THREADS - 1) / IG_THREADS;
+  constexpr int FYS = (TY + 2 * HF) * TZ;
+  constexpr int FZS = TY * (TZ + 2 * HF);
+  constexpr unsigned S = sizeof(T);
   static_assert(RPT * RG == TY, "tile rows must divide");
 
-  __shared__ T s_in[NSLOT * PLANE];
-  __shared__ T s_fy[(TY + 2 * HF) * TZ];
-  __shared__ T s_fz[TY * (TZ + 2 * HF)];
+  __shared__ T s_in[NB][NSLOT * PLANE];
+  __shared__ T s_fy[NB][FYS];
+  __shared__ T s_fz[NB][FZS];
 
-  // XCD-aware block -> (work unit, orbital): consecutive blocks on one XCD share a
-  // work unit so the geometry stream is reused from that XCD's L2.
+  // XCD-aware block -> (work unit, orbital group): consecutive blocks on one XCD share
+  // a work unit so the geometry stream is reused from that XCD's L2.
+  const int ngroups = (p.nblk + NB - 1) / NB;
   const int b = blockIdx.x;
   const int xcd = b & 7;
   const int r = b >> 3;
-  const int orb = r % p.nblk;
-  const int u = (r / p.nblk) * 8 + xcd;
+  const int grp = r % ngroups;
+  const int u = (r / ngroups) * 8 + xcd;
   if (u >= p.units) return;
   const int xci = u % p.nxc;
   const int tzi = (u / p.nxc) % p.ntz;
   const int tyi = u / (p.nxc * p.ntz);
 
   const int nx = p.nx, ny = p.ny, nz = p.nz;
-  const long long nyz = (long long)ny * nz;
+  const int nyz = ny * nz;
+  const unsigned plane_bytes = (unsigned)nyz * S;
   const int j0 = min(tyi * TY, max(ny - TY, 0));
   const int k0 = min(tzi * TZ, max(nz - TZ, 0));
   const int x0 = xci * p.xc;
   const int x1 = min(x0 + p.xc, nx);
-
-  const T* __restrict__ in = p.in + (long long)orb * p.ld;
-  T* __restrict__ out = p.out + (long long)orb * p.ld;
   const long long N = (long long)nx * nyz;
-  const T* __restrict__ cxx = p.coef;
-  const T* __restrict__ cyy = p.coef + N;
-  const T* __restrict__ czz = p.coef + 2 * N;
-  const T* __restrict__ cxy = p.coef + 3 * N;
-  const T* __restrict__ cxz = p.coef + 4 * N;
-  const T* __restrict__ cyz = p.coef + 5 * N;
+
+  rsrc_t rin[NB], rout[NB];
+  bool orb_ok[NB];
+#pragma unroll
+  for (int n = 0; n < NB; ++n) {
+    const int orb = grp * NB + n;
+    orb_ok[n] = orb < p.nblk;
+    const int oc = orb_ok[n] ? orb : p.nblk - 1;
+    rin[n] = mkbuf(p.in + (long long)oc * p.ld, N);
+    rout[n] = mkbuf(p.out + (long long)oc * p.ld, N);
+  }
+  const rsrc_t rxx = mkbuf(p.coef, N);
+  const rsrc_t ryy = mkbuf(p.coef + N, N);
+  const rsrc_t rzz = mkbuf(p.coef + 2 * N, N);
+  const rsrc_t rxy = mkbuf(p.coef + 3 * N, N);
+  const rsrc_t rxz = mkbuf(p.coef + 4 * N, N);
+  const rsrc_t ryz = mkbuf(p.coef + 5 * N, N);
+  const rsrc_t rij = mkbuf(p.invj, N);
+  const rsrc_t rv = mkbuf(p.veff ? p.veff : p.invj, p.veff ? N : 0);  // no V: every load returns 0
 
   const int tid = threadIdx.x;
   const int tz = tid % TZ;
   const int tyb = tid / TZ;
 
-  auto load_plane = [&](int q) {
-    T* dst = s_in + ((q % NSLOT + NSLOT) % NSLOT) * PLANE;
-    const bool qin = (q >= 0 && q < nx);
-    for (int idx = tid; idx < PLANE; idx += IG_THREADS) {
-      const int ly = idx / PZ;
-      const int lz = idx - ly * PZ;
-      const int j = j0 + ly - HP;
-      const int k = k0 + lz - HP;
-      T v = T(0);
-      if (qin && j >= 0 && j < ny && k >= 0 && k < nz) {
-        const long long g = (long long)q * nyz + (long long)j * nz + k;
-        v = in[g];
-        if (TR) v *= p.invj[g];
+  // ---- per-thread constants, computed once: in-plane byte offsets (BAD outside).
+  unsigned poff[NLD];
+#pragma unroll
+  for (int rr = 0; rr < NLD; ++rr) {
+    const int idx = tid + rr * IG_THREADS;
+    const int ly = idx / PZ;
+    const int lz = idx - ly * PZ;
+    const int j = j0 + ly - HP;
+    const int k = k0 + lz - HP;
+    poff[rr] = (idx < PLANE && j >= 0 && j < ny && k >= 0 && k < nz) ? (unsigned)(j * nz + k) * S : BAD;
+  }
+  const bool last_row = (tid + (NLD - 1) * IG_THREADS) < PLANE;
+  unsigned toff[RPT];
+  unsigned tbnd = 0;
+#pragma unroll
+  for (int rr = 0; rr < RPT; ++rr) {
+    const int j = j0 + tyb + rr * RG;
+    const int k = k0 + tz;
+    toff[rr] = (j < ny && k < nz) ? (unsigned)(j * nz + k) * S : BAD;
+    if (j == 0 || j == ny - 1 || k == 0 || k == nz - 1) tbnd |= 1u << rr;
+  }
+  unsigned hoff[NHI];
+  int hkind[NHI];
+  int hj[NHI], hk[NHI], hlds[NHI], hst[NHI];
+#pragma unroll
+  for (int hi = 0; hi < NHI; ++hi) {
+    const int idx = tid + hi * IG_THREADS;
+    int j = j0, k = k0;
+    bool ok = false;
+    hkind[hi] = 0;
+    hlds[hi] = HP * PZ + HP;
+    hst[hi] = 0;
+    if (idx < NHY) {
+      const int hr = idx / TZ;
+      const int cz = idx - hr * TZ;
+      const int fr = hr < HF ? hr : TY + hr;
+      j = j0 + fr - HF;
+      k = k0 + cz;
+      hkind[hi] = 1;
+      hlds[hi] = (fr - HF + HP) * PZ + cz + HP;
+      hst[hi] = fr * TZ + cz;
+      ok = j >= 0 && j < ny && k < nz;
+    } else if (idx < NHY + NHZ) {
+      const int i2 = idx - NHY;
+      const int hcn = i2 / TY;
+      const int ry = i2 - hcn * TY;
+      const int fc = hcn < HF ? hcn : TZ + hcn;
+      j = j0 + ry;
+      k = k0 + fc - HF;
+      hkind[hi] = 2;
+      hlds[hi] = (ry + HP) * PZ + fc - HF + HP;
+      hst[hi] = ry * (TZ + 2 * HF) + fc;
+      ok = j < ny && k >= 0 && k < nz;
+    }
+    hj[hi] = j;
+    hk[hi] = k;
+    hoff[hi] = ok ? (unsigned)(j * nz + k) * S : BAD;
+  }
+
+  // ---- software pipeline registers: each set is refilled right after its last use.
+  T pf[NB][NLD];
+  T gc[RPT][6];
+  T hc[NHI][3];
+  T ic[RPT];
+  T vc[RPT];
+  T xcur[NB][RPT];
+  const T tag = T(0);
+
+  auto pbytes = [&](int q) { return (q >= 0 && q < nx) ? (unsigned)q * plane_bytes : BAD; };
+  auto fetch_plane = [&](int q) {
+    const unsigned pb = pbytes(q);
+#pragma unroll
+    for (int rr = 0; rr < NLD; ++rr) {
+      const unsigned o = pb + poff[rr];
+#pragma unroll
+      for (int n = 0; n < NB; ++n) {
+        pf[n][rr] = bld(rin[n], o, tag);
+        if (TR) pf[n][rr] *= bld(rij, o, tag);
       }
-      dst[idx] = v;
+    }
+  };
+  auto store_plane = [&](int slot) {
+#pragma unroll
+    for (int rr = 0; rr < NLD; ++rr)
+#pragma unroll
+      for (int n = 0; n < NB; ++n)
+        if (rr < NFULL || last_row) s_in[n][slot * PLANE + tid + rr * IG_THREADS] = pf[n][rr];
+  };
+  auto fetch_flux_geom = [&](int q) {
+    const unsigned pb = pbytes(q);
+#pragma unroll
+    for (int rr = 0; rr < RPT; ++rr) {
+      const unsigned o = pb + toff[rr];
+      gc[rr][0] = bld(rxx, o, tag); gc[rr][1] = bld(ryy, o, tag); gc[rr][2] = bld(rzz, o, tag);
+      gc[rr][3] = bld(rxy, o, tag); gc[rr][4] = bld(rxz, o, tag); gc[rr][5] = bld(ryz, o, tag);
+    }
+#pragma unroll
+    for (int hi = 0; hi < NHI; ++hi) {
+      const unsigned o = pb + hoff[hi];
+      const T a_xy = bld(rxy, o, tag), a_yy = bld(ryy, o, tag), a_yz = bld(ryz, o, tag);
+      const T a_xz = bld(rxz, o, tag), a_zz = bld(rzz, o, tag);
+      const bool yrow = hkind[hi] == 1;
+      hc[hi][0] = yrow ? a_xy : a_xz;
+      hc[hi][1] = yrow ? a_yy : a_yz;
+      hc[hi][2] = yrow ? a_yz : a_zz;
+    }
+  };
+  auto fetch_out_geom = [&](int o) {
+    const unsigned pb = pbytes(o);
+#pragma unroll
+    for (int rr = 0; rr < RPT; ++rr) {
+      const unsigned off = pb + toff[rr];
+      ic[rr] = bld(rij, off, tag);
+      vc[rr] = bld(rv, off, tag);
+#pragma unroll
+      for (int n = 0; n < NB; ++n)
+        if (TR) xcur[n][rr] = bld(rin[n], off, tag);
     }
   };
 
-  // Ring slot offsets of planes s-2 .. s+2 (refreshed every step).
-  int sbase[5];
-  // Gradient (scaled) at global (s, j, k) / local (ly, lz) from the ring.
-  auto grad = [&](int s, int j, int k, int ly, int lz, T& gx, T& gy, T& gz) {
-    const T* pl = s_in + sbase[2];
-    const int off = ly * PZ + lz;
-    gx = p.ih2x * dop<CL, TR, T>(s, nx, [&](int d) { return s_in[sbase[2 + d] + off]; });
-    gy = p.ih2y * dop<CL, TR, T>(j, ny, [&](int d) { return pl[off + d * PZ]; });
-    gz = p.ih2z * dop<CL, TR, T>(k, nz, [&](int d) { return pl[off + d]; });
-  };
-
-  T fx[5][RPT];
-  T dyz[3][RPT];
+  // F_x and the y/z divergence of the last planes, indexed by (plane - sb) mod 6.
+  T fxr[NB][NSLOT][RPT];
+  T dyr[NB][NSLOT][RPT];
 #pragma unroll
-  for (int a = 0; a < 5; ++a)
+  for (int n = 0; n < NB; ++n)
 #pragma unroll
-    for (int rr = 0; rr < RPT; ++rr) fx[a][rr] = T(0);
+    for (int a = 0; a < NSLOT; ++a)
 #pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int rr = 0; rr < RPT; ++rr) dyz[a][rr] = T(0);
+      for (int rr = 0; rr < RPT; ++rr) fxr[n][a][rr] = dyr[n][a][rr] = T(0);
 
   const int sb = x0 - 2;
   const int se = x1 + 2;
-  for (int q = sb - 2; q < sb + 2; ++q) load_plane(q);
+  // ring slot of plane q: (q - sb + 2) mod 6
+  for (int q = sb - 2; q < sb + 2; ++q) {
+    fetch_plane(q);
+    store_plane(q - sb + 2);
+  }
+  fetch_plane(sb + 2);
+  fetch_flux_geom(sb);
+  fetch_out_geom(sb - 2);
+  const T sc = p.scale;
 
-  for (int s = sb; s < se; ++s) {
-    // Phase A: prefetch plane s+2 into the slot of plane s-4.
-    load_plane(s + 2);
+  for (int s0 = sb; s0 < se; s0 += NSLOT) {
+    // Unrolled by the ring period: every slot and register index below is a constant.
 #pragma unroll
-    for (int d = 0; d < 5; ++d) sbase[d] = (((s - 2 + d) % NSLOT + NSLOT) % NSLOT) * PLANE;
-    __syncthreads();
+    for (int U = 0; U < NSLOT; ++U) {
+      const int s = s0 + U;
+      // Phase A: commit plane s+2, request plane s+3.
+      store_plane((U + 4) % NSLOT);
+      fetch_plane(s + 3);
+      __syncthreads();
 
-    // Phase B: fluxes of plane s.
-    const bool sin_ = (s >= 0 && s < nx);
+      // Phase B: fluxes of plane s (coefficients are zero outside the domain).
 #pragma unroll
-    for (int rr = 0; rr < RPT; ++rr) {
-      const int ty = tyb + rr * RG;
-      const int j = j0 + ty;
-      const int k = k0 + tz;
-      T Fx = T(0), Fy = T(0), Fz = T(0);
-      if (sin_ && j < ny && k < nz) {
-        T gx, gy, gz;
-        grad(s, j, k, ty + HP, tz + HP, gx, gy, gz);
-        const long long g = (long long)s * nyz + (long long)j * nz + k;
-        const T a_xx = cxx[g], a_yy = cyy[g], a_zz = czz[g];
-        const T a_xy = cxy[g], a_xz = cxz[g], a_yz = cyz[g];
-        if (TR) {
-          // F_b = sum_a c^{ab} G_a with G = D^T u; c symmetric so same form.
-          Fx = a_xx * gx + a_xy * gy + a_xz * gz;
-          Fy = a_xy * gx + a_yy * gy + a_yz * gz;
-          Fz = a_xz * gx + a_yz * gy + a_zz * gz;
-        } else {
-          Fx = a_xx * gx + a_xy * gy + a_xz * gz;
-          Fy = a_xy * gx + a_yy * gy + a_yz * gz;
-          Fz = a_xz * gx + a_yz * gy + a_zz * gz;
+      for (int n = 0; n < NB; ++n) {
+        const T* ring = s_in[n];
+        auto grad = [&](int j, int k, int off, T& gx, T& gy, T& gz) {
+          const T* pl = ring + ((U + 2) % NSLOT) * PLANE;
+          gx = dop<CL, TR, T>(s, nx, [&](int d) { return ring[((U + 2 + d + NSLOT) % NSLOT) * PLANE + off]; });
+          gy = dop<CL, TR, T>(j, ny, [&](int d) { return pl[off + d * PZ]; });
+          gz = dop<CL, TR, T>(k, nz, [&](int d) { return pl[off + d]; });
+        };
+#pragma unroll
+        for (int rr = 0; rr < RPT; ++rr) {
+          const int ty = tyb + rr * RG;
+          T gx, gy, gz;
+          grad(j0 + ty, k0 + tz, (ty + HP) * PZ + tz + HP, gx, gy, gz);
+          fxr[n][U][rr] = flux3(gc[rr][0], gx, gc[rr][3], gy, gc[rr][4], gz);
+          s_fy[n][(ty + HF) * TZ + tz] = flux3(gc[rr][3], gx, gc[rr][1], gy, gc[rr][5], gz);
+          s_fz[n][ty * (TZ + 2 * HF) + tz + HF] = flux3(gc[rr][4], gx, gc[rr][5], gy, gc[rr][2], gz);
+        }
+#pragma unroll
+        for (int hi = 0; hi < NHI; ++hi) {
+          T gx, gy, gz;
+          grad(hj[hi], hk[hi], hlds[hi], gx, gy, gz);
+          const T F = flux3(hc[hi][0], gx, hc[hi][1], gy, hc[hi][2], gz);
+          if (hkind[hi] == 1) s_fy[n][hst[hi]] = F;
+          if (hkind[hi] == 2) s_fz[n][hst[hi]] = F;
         }
       }
+      fetch_flux_geom(s + 1);
+      __syncthreads();
+
+      // Phase C: y/z divergence of plane s, then output plane o = s - 2.
+      const int o = s - 2;
+      const bool emit = (o >= x0 && o < x1);
+      const int oc = min(max(o, 0), nx - 1);
+      const bool obnd = (o == 0 || o == nx - 1);
+      const unsigned ob = emit ? (unsigned)o * plane_bytes : BAD;
 #pragma unroll
-      for (int a = 0; a < 4; ++a) fx[a][rr] = fx[a + 1][rr];
-      fx[4][rr] = Fx;
-      s_fy[(ty + HF) * TZ + tz] = Fy;
-      s_fz[ty * (TZ + 2 * HF) + tz + HF] = Fz;
-    }
-    // y-halo rows of F_y and z-halo columns of F_z.
-    {
-      constexpr int NHY = 2 * HF * TZ;
-      constexpr int NHZ = 2 * HF * TY;
-      for (int idx = tid; idx < NHY + NHZ; idx += IG_THREADS) {
-        if (idx < NHY) {
-          const int hr = idx / TZ;           // 0 .. 2HF-1
-          const int cz = idx - hr * TZ;
-          const int fr = hr < HF ? hr : TY + hr;  // F row index in s_fy
-          const int j = j0 + fr - HF;
-          const int k = k0 + cz;
-          T F = T(0);
-          if (sin_ && j >= 0 && j < ny && k < nz) {
-            T gx, gy, gz;
-            grad(s, j, k, fr - HF + HP, cz + HP, gx, gy, gz);
-            const long long g = (long long)s * nyz + (long long)j * nz + k;
-            F = cxy[g] * gx + cyy[g] * gy + cyz[g] * gz;
+      for (int n = 0; n < NB; ++n) {
+        const unsigned obn = orb_ok[n] ? ob : BAD;
+#pragma unroll
+        for (int rr = 0; rr < RPT; ++rr) {
+          const int ty = tyb + rr * RG;
+          const int fy0 = (ty + HF) * TZ + tz;
+          const int fz0 = ty * (TZ + 2 * HF) + tz + HF;
+          dyr[n][U][rr] = dop<CL, TR, T>(j0 + ty, ny, [&](int dd) { return s_fy[n][fy0 + dd * TZ]; }) +
+                          dop<CL, TR, T>(k0 + tz, nz, [&](int dd) { return s_fz[n][fz0 + dd]; });
+          const T dx = dop<CL, TR, T>(oc, nx, [&](int dd) { return fxr[n][(U - 2 + dd + NSLOT) % NSLOT][rr]; });
+          const T lap = dx + dyr[n][(U - 2 + NSLOT) % NSLOT][rr];
+          T v;
+          if (TR) {
+            v = fma(vc[rr], xcur[n][rr], sc * lap);
+          } else {
+            const T psi = s_in[n][U * PLANE + (ty + HP) * PZ + tz + HP];
+            v = fma(vc[rr], psi, sc * (lap * ic[rr]));
           }
-          s_fy[fr * TZ + cz] = F;
-        } else {
-          const int i2 = idx - NHY;
-          const int hc = i2 / TY;
-          const int ry = i2 - hc * TY;
-          const int fc = hc < HF ? hc : TZ + hc;  // F column index in s_fz
-          const int j = j0 + ry;
-          const int k = k0 + fc - HF;
-          T F = T(0);
-          if (sin_ && j < ny && k >= 0 && k < nz) {
-            T gx, gy, gz;
-            grad(s, j, k, ry + HP, fc - HF + HP, gx, gy, gz);
-            const long long g = (long long)s * nyz + (long long)j * nz + k;
-            F = cxz[g] * gx + cyz[g] * gy + czz[g] * gz;
-          }
-          s_fz[ry * (TZ + 2 * HF) + fc] = F;
+          if (p.interior_only && (obnd || ((tbnd >> rr) & 1u))) v = T(0);
+          bst(rout[n], obn + toff[rr], v);
         }
       }
-    }
-    __syncthreads();
-
-    // Phase C: y/z divergence of plane s, then output plane o = s - 2.
-    const int o = s - 2;
-    const bool emit = (o >= x0 && o < x1);
-#pragma unroll
-    for (int rr = 0; rr < RPT; ++rr) {
-      const int ty = tyb + rr * RG;
-      const int j = j0 + ty;
-      const int k = k0 + tz;
-      T d = T(0);
-      if (sin_ && j < ny && k < nz) {
-        const int fy0 = (ty + HF) * TZ + tz;
-        const int fz0 = ty * (TZ + 2 * HF) + tz + HF;
-        d = p.ih2y * dop<CL, TR, T>(j, ny, [&](int dd) { return s_fy[fy0 + dd * TZ]; }) +
-            p.ih2z * dop<CL, TR, T>(k, nz, [&](int dd) { return s_fz[fz0 + dd]; });
-      }
-      dyz[2][rr] = dyz[1][rr];
-      dyz[1][rr] = dyz[0][rr];
-      dyz[0][rr] = d;
-      if (emit && j < ny && k < nz) {
-        const T dx = p.ih2x * dop<CL, TR, T>(o, nx, [&](int dd) { return fx[2 + dd][rr]; });
-        const long long g = (long long)o * nyz + (long long)j * nz + k;
-        const T lap = dx + dyz[2][rr];
-        T v;
-        if (TR) {
-          const T xin = in[g];
-          v = p.scale * lap;
-          if (p.veff) v += p.veff[g] * xin;
-        } else {
-          const T psi = s_in[(o % NSLOT) * PLANE + (ty + HP) * PZ + tz + HP];
-          v = p.scale * (lap * p.invj[g]);
-          if (p.veff) v += p.veff[g] * psi;
-        }
-        if (p.interior_only &&
-            (o == 0 || o == nx - 1 || j == 0 || j == ny - 1 || k == 0 || k == nz - 1))
-          v = T(0);
-        out[g] = v;
-      }
+      fetch_out_geom(s - 1);
     }
   }
 }
 
-template <typename T, int TY, int TZ, int CL, bool TR>
+template <typename T, int TY, int TZ, int CL, bool TR, int NB = 1>
 int launch_monitor_t(const MonParams<T>& base, hipStream_t stream) {
   MonParams<T> p = base;
+  // buffer-resource addressing: every field must stay below 1 GiB (the BAD offset)
+  if ((long long)p.nx * p.ny * p.nz * (long long)sizeof(T) >= (1LL << 30)) return IG_EUNSUPPORTED;
   p.nty = (p.ny + TY - 1) / TY;
   p.ntz = (p.nz + TZ - 1) / TZ;
-  if (p.xc <= 0 || p.xc > p.nx) p.xc = p.nx;
+  if (p.xc <= 0) {
+    // Auto: split x so that at least ~2048 workgroups are in flight (8 per CU on a
+    // 256-CU MI355X), but never below 16 planes per chunk (each chunk re-reads 8 planes).
+    const long long base = (long long)p.nty * p.ntz * ((p.nblk + NB - 1) / NB);
+    long long nxc = (2048 + base - 1) / base;
+    const long long maxc = p.nx / 16 > 1 ? p.nx / 16 : 1;
+    if (nxc > maxc) nxc = maxc;
+    if (nxc < 1) nxc = 1;
+    p.xc = (int)((p.nx + nxc - 1) / nxc);
+  }
+  if (p.xc > p.nx) p.xc = p.nx;
   p.nxc = (p.nx + p.xc - 1) / p.xc;
   p.units = p.nty * p.ntz * p.nxc;
-  const long long blocks = 8LL * p.nblk * ((p.units + 7) / 8);
+  const long long blocks = 8LL * ((p.nblk + NB - 1) / NB) * ((p.units + 7) / 8);
   if (blocks <= 0 || blocks > 0x7fffffffLL) return IG_EINVAL;
-  hipLaunchKernelGGL((monitor_stencil_kernel<T, TY, TZ, CL, TR>), dim3((unsigned)blocks),
+  hipLaunchKernelGGL((monitor_stencil_kernel<T, TY, TZ, CL, TR, NB>), dim3((unsigned)blocks),
                      dim3(IG_THREADS), 0, stream, p);
   IG_CHECK_LAUNCH();
   return IG_OK;
 }
 
+#define IG_MON_CASE(TY_, TZ_, NB_)                                                                 \
+  if (ty == TY_ && tz == TZ_ && nb == NB_)                                                         \
+    return closure ? launch_monitor_t<T, TY_, TZ_, 1, false, NB_>(p, s)                            \
+                   : launch_monitor_t<T, TY_, TZ_, 0, false, NB_>(p, s);
+
+// tile: 0 = automatic; 32 / 64 = z tile width with the default rows; otherwise
+// NB*10000 + TZ*100 + TY (e.g. 23216: two orbitals per workgroup, 16 x 32 tiles).
 template <typename T>
 int launch_monitor(const MonParams<T>& p, int closure, int transpose, int tile, hipStream_t s) {
-  // tile: 0 = auto; otherwise TZ (32 or 64)
-  int tz = tile;
-  if (tz == 0) tz = (p.nz % 64 == 0 || p.nz < 48) ? 64 : 32;
-  if (sizeof(T) == 8) {
-    if (transpose) tz = 32;  // keeps the transposed FP64 tile under 64 KB of LDS
-    if (tz == 64) {
-      if (transpose) return closure ? launch_monitor_t<T, 8, 64, 1, true>(p, s) : launch_monitor_t<T, 8, 64, 0, true>(p, s);
-      return closure ? launch_monitor_t<T, 8, 64, 1, false>(p, s) : launch_monitor_t<T, 8, 64, 0, false>(p, s);
-    }
-    if (transpose) return closure ? launch_monitor_t<T, 8, 32, 1, true>(p, s) : launch_monitor_t<T, 8, 32, 0, true>(p, s);
-    return closure ? launch_monitor_t<T, 8, 32, 1, false>(p, s) : launch_monitor_t<T, 8, 32, 0, false>(p, s);
-  }
-  if (tz == 64) {
-    if (transpose) return closure ? launch_monitor_t<T, 8, 64, 1, true>(p, s) : launch_monitor_t<T, 8, 64, 0, true>(p, s);
-    return closure ? launch_monitor_t<T, 16, 64, 1, false>(p, s) : launch_monitor_t<T, 16, 64, 0, false>(p, s);
-  }
   if (transpose) return closure ? launch_monitor_t<T, 8, 32, 1, true>(p, s) : launch_monitor_t<T, 8, 32, 0, true>(p, s);
-  return closure ? launch_monitor_t<T, 16, 32, 1, false>(p, s) : launch_monitor_t<T, 16, 32, 0, false>(p, s);
+  int nb = tile >= 10000 ? tile / 10000 : 0;
+  const int t2 = tile % 10000;
+  int tz = t2 >= 100 ? t2 / 100 : t2;
+  int ty = t2 >= 100 ? t2 % 100 : 0;
+  if (tz == 0) tz = 32;
+  if (ty == 0) ty = (sizeof(T) == 8) ? (tz == 64 ? 8 : 16) : 16;
+  if (nb == 0) nb = (p.nblk >= 2 && tz == 32) ? 2 : 1;
+  if (nb > p.nblk) nb = (p.nblk >= 2 && tz == 32) ? 2 : 1;
+  if constexpr (sizeof(T) == 8) {
+    IG_MON_CASE(8, 64, 1)
+    IG_MON_CASE(8, 32, 1)
+    IG_MON_CASE(16, 32, 1)
+    IG_MON_CASE(8, 32, 2)
+    IG_MON_CASE(16, 32, 2)
+  } else {
+    IG_MON_CASE(8, 64, 1)
+    IG_MON_CASE(16, 64, 1)
+    IG_MON_CASE(8, 32, 1)
+    IG_MON_CASE(16, 32, 1)
+    IG_MON_CASE(32, 32, 1)
+    IG_MON_CASE(8, 32, 2)
+    IG_MON_CASE(16, 32, 2)
+    IG_MON_CASE(8, 32, 4)
+    IG_MON_CASE(16, 32, 4)
+  }
+  return IG_EUNSUPPORTED;
 }
+#undef IG_MON_CASE
 
 // ------------------------------------------------------------------ legacy 7-point
 
@@ -387,13 +528,13 @@ int launch_legacy(const LegParams<T>& p, hipStream_t s) {
 
 template <typename T>
 int monitor_entry(const T* in, T* out, int nblk, long long ld, int nx, int ny, int nz,
-                  double hx, double hy, double hz, const T* coef, const T* invj, const T* veff,
+                  const T* coef, const T* invj, const T* veff,
                   double scale, int closure, int transpose, int interior_only, int xchunk,
                   int tile, hipStream_t stream) {
   if (!in || !out || !coef || !invj || nblk <= 0 || nx < 3 || ny < 3 || nz < 3) return IG_EINVAL;
   if (ld < (long long)nx * ny * nz) return IG_EINVAL;
   if (closure < 0 || closure > 1) return IG_EINVAL;
-  if (tile != 0 && tile != 32 && tile != 64) return IG_EINVAL;
+  if (tile < 0 || (tile != 0 && tile != 32 && tile != 64 && tile < 100)) return IG_EINVAL;
   MonParams<T> p{};
   p.in = in;
   p.out = out;
@@ -405,9 +546,6 @@ int monitor_entry(const T* in, T* out, int nblk, long long ld, int nx, int ny, i
   p.nx = nx;
   p.ny = ny;
   p.nz = nz;
-  p.ih2x = T(0.5 / hx);
-  p.ih2y = T(0.5 / hy);
-  p.ih2z = T(0.5 / hz);
   p.scale = T(scale);
   p.interior_only = interior_only;
   p.xc = xchunk;
@@ -446,20 +584,18 @@ int legacy_entry(const T* in, T* out, int nblk, long long ld, int nx, int ny, in
 extern "C" {
 
 int ig_monitor_stencil_f64(const double* in, double* out, int nblk, long long ld, int nx, int ny,
-                           int nz, double hx, double hy, double hz, const double* coef,
-                           const double* invj, const double* veff, double scale, int closure,
-                           int transpose, int interior_only, int xchunk, int tile,
-                           hipStream_t stream) {
-  return monitor_entry<double>(in, out, nblk, ld, nx, ny, nz, hx, hy, hz, coef, invj, veff, scale,
+                           int nz, const double* coef, const double* invj, const double* veff,
+                           double scale, int closure, int transpose, int interior_only,
+                           int xchunk, int tile, hipStream_t stream) {
+  return monitor_entry<double>(in, out, nblk, ld, nx, ny, nz, coef, invj, veff, scale,
                                closure, transpose, interior_only, xchunk, tile, stream);
 }
 
 int ig_monitor_stencil_f32(const float* in, float* out, int nblk, long long ld, int nx, int ny,
-                           int nz, double hx, double hy, double hz, const float* coef,
-                           const float* invj, const float* veff, double scale, int closure,
-                           int transpose, int interior_only, int xchunk, int tile,
-                           hipStream_t stream) {
-  return monitor_entry<float>(in, out, nblk, ld, nx, ny, nz, hx, hy, hz, coef, invj, veff, scale,
+                           int nz, const float* coef, const float* invj, const float* veff,
+                           double scale, int closure, int transpose, int interior_only,
+                           int xchunk, int tile, hipStream_t stream) {
+  return monitor_entry<float>(in, out, nblk, ld, nx, ny, nz, coef, invj, veff, scale,
                               closure, transpose, interior_only, xchunk, tile, stream);
 }
 

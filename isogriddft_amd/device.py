@@ -79,6 +79,11 @@ class DeviceGrid:
             lx, ly, lz = _grid.logical_axes(geom)
             self.h = (_grid.uniform_spacing(lx, "x"), _grid.uniform_spacing(ly, "y"),
                       _grid.uniform_spacing(lz, "z"))
+            # stencil-scaled coefficients c^{ab} / (4 h_a h_b), the kernels' operand
+            hx, hy, hz = self.h
+            fac = np.array([1 / (4 * hx * hx), 1 / (4 * hy * hy), 1 / (4 * hz * hz),
+                            1 / (4 * hx * hy), 1 / (4 * hx * hz), 1 / (4 * hy * hz)])
+            self._t[("scoef", f64)] = torch.from_numpy(np.ascontiguousarray(coef * fac[:, None])).to(dev)
             self.box_bounds = tuple(tuple(b) for b in _grid.box_bounds(geom))
         else:
             parts = []

@@ -76,14 +76,13 @@ def apply_operator(
     if dg.kind == "monitor":
         cl = _closure_id(closure)
         _lib.call(f"ig_monitor_stencil_{sfx}", _lib.ptr(xb), _lib.ptr(ob), nblk, dg.N, nx, ny, nz,
-                  dg.h[0], dg.h[1], dg.h[2], _lib.ptr(dg.coef(dt)), _lib.ptr(dg.invj(dt)),
-                  _lib.ptr(veff), float(scale), cl, int(transpose), int(interior_only), int(xchunk),
-                  int(tile), st)
+                  _lib.ptr(dg.get("scoef", dt)), _lib.ptr(dg.invj(dt)), _lib.ptr(veff), float(scale), cl,
+                  int(transpose), int(interior_only), int(xchunk), int(tile), st)
     else:
         _lib.call(f"ig_legacy_stencil_{sfx}", _lib.ptr(xb), _lib.ptr(ob), nblk, dg.N, nx, ny, nz,
                   _lib.ptr(dg.coef(dt)), _lib.ptr(veff), float(scale), int(transpose),
                   int(interior_only), st)
-    return out.reshape(x.shape) if out.shape == x.shape else out
+    return out.reshape(x.shape) if out.numel() == x.numel() else out
 
 
 def _run(psi, grid_geometry, scale, closure):

@@ -43,7 +43,7 @@ def test_host_only_entry_points():
 def test_invalid_arguments_are_rejected_before_any_launch():
     # NULL device pointers are rejected on the host side with status -1 -> ValueError
     with pytest.raises(ValueError):
-        _lib.call("ig_monitor_stencil_f64", None, None, 1, 1000, 10, 10, 10, 0.1, 0.1, 0.1,
+        _lib.call("ig_monitor_stencil_f64", None, None, 1, 1000, 10, 10, 10,
                   None, None, None, 1.0, 1, 0, 0, 0, 0, None)
     with pytest.raises(ValueError):
         _lib.call("ig_gram_f32", None, 10, 2, None, 10, 2, None, 10, None, 0, None, 0, None)
