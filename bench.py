@@ -47,6 +47,8 @@ def parse():
                     help="c2: 96^3, k=16 (default); c5: 224^3, k=48 (benzene-size block)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 256 MiB device copy (the scale for PMC byte counters)")
     return ap.parse_args()
 
 
@@ -113,6 +115,13 @@ def main():
         kinetic.apply_operator(dg, ins[i % R], outs[i % R], scale=-0.5, closure=args.closure, veff=veff,
                                tile=args.tile, xchunk=args.xchunk)
 
+    if args.copy_calibration:
+        src = torch.ones(64 * 2 ** 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for i in range(args.warmup):
         step(i)
     torch.cuda.synchronize()

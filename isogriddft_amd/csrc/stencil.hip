@@ -107,7 +107,7 @@ __device__ __forceinline__ void bst(rsrc_t r, unsigned off, double v) {
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)off, 0, 0);
 }
 
-template <typename T, int TY, int TZ, int CL, bool TR, int NB>
+template <typename T, int TY, int TZ, int CL, bool TR, int NB, int PD>
 __global__ __launch_bounds__(IG_THREADS) void monitor_stencil_kernel(MonParams<T> p) {
   constexpr int HF = TR ? 2 : 1;  // flux halo
   constexpr int HP = 2 * HF;      // input halo
@@ -118,7 +118,6 @@ __global__ __launch_bounds__(IG_THREADS) void monitor_stencil_kernel(MonParams<T
   constexpr int RG = IG_THREADS / TZ;  // row groups
   constexpr int RPT = TY / RG;         // rows per thread
   constexpr int NLD = (PLANE + IG_THREADS - 1) / IG_THREADS;  // ring elements per thread
-  constexpr int NFULL = PLANE / IG_THREADS;
   constexpr int NHY = 2 * HF * TZ;     // y-halo flux points
   constexpr int NHZ = 2 * HF * TY;     // z-halo flux points
   constexpr int NHI = (NHY + NHZ + IG_THREADS - 1) / IG_THREADS;
@@ -126,23 +125,30 @@ __global__ __launch_bounds__(IG_THREADS) void monitor_stencil_kernel(MonParams<T
   constexpr int FZS = TY * (TZ + 2 * HF);
   constexpr unsigned S = sizeof(T);
   static_assert(RPT * RG == TY, "tile rows must divide");
+  static_assert(PD >= 1 && NSLOT % PD == 0, "prefetch depth must divide the ring period");
 
-  __shared__ T s_in[NB][NSLOT * PLANE];
-  __shared__ T s_fy[NB][FYS];
-  __shared__ T s_fz[NB][FZS];
+  constexpr int PSTR = NLD * IG_THREADS;  // ring slot stride: every thread owns NLD entries
+  __shared__ T s_in[NB][NSLOT * PSTR];
+  // y fluxes [FYS], z fluxes [FZS] and one spare entry that absorbs the stores of
+  // threads without a halo point, so no store is predicated.
+  __shared__ T s_fl[NB][FYS + FZS + 1];
 
-  // XCD-aware block -> (work unit, orbital group): consecutive blocks on one XCD share
-  // a work unit so the geometry stream is reused from that XCD's L2.
+  // XCD-aware block -> (work unit, orbital group).  Workgroup b runs on XCD b % 8, so
+  // each XCD takes one contiguous range of work units (z tiles fastest, then y tiles,
+  // then x chunks): tiles that share halo rows and cache lines run together on one L2.
+  // Within an XCD the orbital groups of a unit are consecutive, so they also share
+  // that unit's geometry stream from L2.
   const int ngroups = (p.nblk + NB - 1) / NB;
   const int b = blockIdx.x;
   const int xcd = b & 7;
   const int r = b >> 3;
   const int grp = r % ngroups;
-  const int u = (r / ngroups) * 8 + xcd;
+  const int upx = (p.units + 7) / 8;
+  const int u = xcd * upx + r / ngroups;
   if (u >= p.units) return;
-  const int xci = u % p.nxc;
-  const int tzi = (u / p.nxc) % p.ntz;
-  const int tyi = u / (p.nxc * p.ntz);
+  const int tzi = u % p.ntz;
+  const int tyi = (u / p.ntz) % p.nty;
+  const int xci = u / (p.ntz * p.nty);
 
   const int nx = p.nx, ny = p.ny, nz = p.nz;
   const int nyz = ny * nz;
@@ -187,7 +193,6 @@ __global__ __launch_bounds__(IG_THREADS) void monitor_stencil_kernel(MonParams<T
     const int k = k0 + lz - HP;
     poff[rr] = (idx < PLANE && j >= 0 && j < ny && k >= 0 && k < nz) ? (unsigned)(j * nz + k) * S : BAD;
   }
-  const bool last_row = (tid + (NLD - 1) * IG_THREADS) < PLANE;
   unsigned toff[RPT];
   unsigned tbnd = 0;
 #pragma unroll
@@ -207,7 +212,7 @@ __global__ __launch_bounds__(IG_THREADS) void monitor_stencil_kernel(MonParams<T
     bool ok = false;
     hkind[hi] = 0;
     hlds[hi] = HP * PZ + HP;
-    hst[hi] = 0;
+    hst[hi] = FYS + FZS;
     if (idx < NHY) {
       const int hr = idx / TZ;
       const int cz = idx - hr * TZ;
@@ -227,7 +232,7 @@ __global__ __launch_bounds__(IG_THREADS) void monitor_stencil_kernel(MonParams<T
       k = k0 + fc - HF;
       hkind[hi] = 2;
       hlds[hi] = (ry + HP) * PZ + fc - HF + HP;
-      hst[hi] = ry * (TZ + 2 * HF) + fc;
+      hst[hi] = FYS + ry * (TZ + 2 * HF) + fc;
       ok = j < ny && k >= 0 && k < nz;
     }
     hj[hi] = j;
@@ -235,42 +240,43 @@ __global__ __launch_bounds__(IG_THREADS) void monitor_stencil_kernel(MonParams<T
     hoff[hi] = ok ? (unsigned)(j * nz + k) * S : BAD;
   }
 
-  // ---- software pipeline registers: each set is refilled right after its last use.
-  T pf[NB][NLD];
-  T gc[RPT][6];
-  T hc[NHI][3];
-  T ic[RPT];
-  T vc[RPT];
-  T xcur[NB][RPT];
+  // ---- software pipeline registers, PD planes deep: each set is refilled with the
+  // plane PD steps ahead right after its last use.
+  T pf[PD][NB][NLD];
+  T gc[PD][RPT][6];
+  T hc[PD][NHI][3];
+  T ic[PD][RPT];
+  T vc[PD][RPT];
+  T xcur[PD][NB][RPT];
   const T tag = T(0);
 
   auto pbytes = [&](int q) { return (q >= 0 && q < nx) ? (unsigned)q * plane_bytes : BAD; };
-  auto fetch_plane = [&](int q) {
+  auto fetch_plane = [&](int q, int w) {
     const unsigned pb = pbytes(q);
 #pragma unroll
     for (int rr = 0; rr < NLD; ++rr) {
       const unsigned o = pb + poff[rr];
 #pragma unroll
       for (int n = 0; n < NB; ++n) {
-        pf[n][rr] = bld(rin[n], o, tag);
-        if (TR) pf[n][rr] *= bld(rij, o, tag);
+        pf[w][n][rr] = bld(rin[n], o, tag);
+        if (TR) pf[w][n][rr] *= bld(rij, o, tag);
       }
     }
   };
-  auto store_plane = [&](int slot) {
+  auto store_plane = [&](int slot, int w) {
 #pragma unroll
     for (int rr = 0; rr < NLD; ++rr)
 #pragma unroll
       for (int n = 0; n < NB; ++n)
-        if (rr < NFULL || last_row) s_in[n][slot * PLANE + tid + rr * IG_THREADS] = pf[n][rr];
+        s_in[n][slot * PSTR + tid + rr * IG_THREADS] = pf[w][n][rr];
   };
-  auto fetch_flux_geom = [&](int q) {
+  auto fetch_flux_geom = [&](int q, int w) {
     const unsigned pb = pbytes(q);
 #pragma unroll
     for (int rr = 0; rr < RPT; ++rr) {
       const unsigned o = pb + toff[rr];
-      gc[rr][0] = bld(rxx, o, tag); gc[rr][1] = bld(ryy, o, tag); gc[rr][2] = bld(rzz, o, tag);
-      gc[rr][3] = bld(rxy, o, tag); gc[rr][4] = bld(rxz, o, tag); gc[rr][5] = bld(ryz, o, tag);
+      gc[w][rr][0] = bld(rxx, o, tag); gc[w][rr][1] = bld(ryy, o, tag); gc[w][rr][2] = bld(rzz, o, tag);
+      gc[w][rr][3] = bld(rxy, o, tag); gc[w][rr][4] = bld(rxz, o, tag); gc[w][rr][5] = bld(ryz, o, tag);
     }
 #pragma unroll
     for (int hi = 0; hi < NHI; ++hi) {
@@ -278,21 +284,21 @@ __global__ __launch_bounds__(IG_THREADS) void monitor_stencil_kernel(MonParams<T
       const T a_xy = bld(rxy, o, tag), a_yy = bld(ryy, o, tag), a_yz = bld(ryz, o, tag);
       const T a_xz = bld(rxz, o, tag), a_zz = bld(rzz, o, tag);
       const bool yrow = hkind[hi] == 1;
-      hc[hi][0] = yrow ? a_xy : a_xz;
-      hc[hi][1] = yrow ? a_yy : a_yz;
-      hc[hi][2] = yrow ? a_yz : a_zz;
+      hc[w][hi][0] = yrow ? a_xy : a_xz;
+      hc[w][hi][1] = yrow ? a_yy : a_yz;
+      hc[w][hi][2] = yrow ? a_yz : a_zz;
     }
   };
-  auto fetch_out_geom = [&](int o) {
+  auto fetch_out_geom = [&](int o, int w) {
     const unsigned pb = pbytes(o);
 #pragma unroll
     for (int rr = 0; rr < RPT; ++rr) {
       const unsigned off = pb + toff[rr];
-      ic[rr] = bld(rij, off, tag);
-      vc[rr] = bld(rv, off, tag);
+      ic[w][rr] = bld(rij, off, tag);
+      vc[w][rr] = bld(rv, off, tag);
 #pragma unroll
       for (int n = 0; n < NB; ++n)
-        if (TR) xcur[n][rr] = bld(rin[n], off, tag);
+        if (TR) xcur[w][n][rr] = bld(rin[n], off, tag);
     }
   };
 
@@ -308,14 +314,19 @@ __global__ __launch_bounds__(IG_THREADS) void monitor_stencil_kernel(MonParams<T
 
   const int sb = x0 - 2;
   const int se = x1 + 2;
-  // ring slot of plane q: (q - sb + 2) mod 6
+  // LDS ring slot of plane q: (q - sb + 2) mod 6.  Register queue slots: psi plane q in
+  // (q - sb - 2) mod PD, flux geometry of plane q in (q - sb) mod PD, output geometry
+  // of plane q in (q - sb + 2) mod PD; 6 % PD == 0 keeps them constant when unrolled.
   for (int q = sb - 2; q < sb + 2; ++q) {
-    fetch_plane(q);
-    store_plane(q - sb + 2);
+    fetch_plane(q, 0);
+    store_plane(q - sb + 2, 0);
   }
-  fetch_plane(sb + 2);
-  fetch_flux_geom(sb);
-  fetch_out_geom(sb - 2);
+#pragma unroll
+  for (int d = 0; d < PD; ++d) {
+    fetch_plane(sb + 2 + d, d);
+    fetch_flux_geom(sb + d, d);
+    fetch_out_geom(sb - 2 + d, d);
+  }
   const T sc = p.scale;
 
   for (int s0 = sb; s0 < se; s0 += NSLOT) {
@@ -323,9 +334,10 @@ __global__ __launch_bounds__(IG_THREADS) void monitor_stencil_kernel(MonParams<T
 #pragma unroll
     for (int U = 0; U < NSLOT; ++U) {
       const int s = s0 + U;
-      // Phase A: commit plane s+2, request plane s+3.
-      store_plane((U + 4) % NSLOT);
-      fetch_plane(s + 3);
+      const int W = U % PD;  // a constant once the loop is unrolled
+      // Phase A: commit plane s+2, request plane s+2+PD.
+      store_plane((U + 4) % NSLOT, W);
+      fetch_plane(s + 2 + PD, W);
       __syncthreads();
 
       // Phase B: fluxes of plane s (coefficients are zero outside the domain).
@@ -333,8 +345,8 @@ __global__ __launch_bounds__(IG_THREADS) void monitor_stencil_kernel(MonParams<T
       for (int n = 0; n < NB; ++n) {
         const T* ring = s_in[n];
         auto grad = [&](int j, int k, int off, T& gx, T& gy, T& gz) {
-          const T* pl = ring + ((U + 2) % NSLOT) * PLANE;
-          gx = dop<CL, TR, T>(s, nx, [&](int d) { return ring[((U + 2 + d + NSLOT) % NSLOT) * PLANE + off]; });
+          const T* pl = ring + ((U + 2) % NSLOT) * PSTR;
+          gx = dop<CL, TR, T>(s, nx, [&](int d) { return ring[((U + 2 + d + NSLOT) % NSLOT) * PSTR + off]; });
           gy = dop<CL, TR, T>(j, ny, [&](int d) { return pl[off + d * PZ]; });
           gz = dop<CL, TR, T>(k, nz, [&](int d) { return pl[off + d]; });
         };
@@ -343,20 +355,19 @@ __global__ __launch_bounds__(IG_THREADS) void monitor_stencil_kernel(MonParams<T
           const int ty = tyb + rr * RG;
           T gx, gy, gz;
           grad(j0 + ty, k0 + tz, (ty + HP) * PZ + tz + HP, gx, gy, gz);
-          fxr[n][U][rr] = flux3(gc[rr][0], gx, gc[rr][3], gy, gc[rr][4], gz);
-          s_fy[n][(ty + HF) * TZ + tz] = flux3(gc[rr][3], gx, gc[rr][1], gy, gc[rr][5], gz);
-          s_fz[n][ty * (TZ + 2 * HF) + tz + HF] = flux3(gc[rr][4], gx, gc[rr][5], gy, gc[rr][2], gz);
+          fxr[n][U][rr] = flux3(gc[W][rr][0], gx, gc[W][rr][3], gy, gc[W][rr][4], gz);
+          s_fl[n][(ty + HF) * TZ + tz] = flux3(gc[W][rr][3], gx, gc[W][rr][1], gy, gc[W][rr][5], gz);
+          s_fl[n][FYS + ty * (TZ + 2 * HF) + tz + HF] = flux3(gc[W][rr][4], gx, gc[W][rr][5], gy, gc[W][rr][2], gz);
         }
 #pragma unroll
         for (int hi = 0; hi < NHI; ++hi) {
           T gx, gy, gz;
           grad(hj[hi], hk[hi], hlds[hi], gx, gy, gz);
-          const T F = flux3(hc[hi][0], gx, hc[hi][1], gy, hc[hi][2], gz);
-          if (hkind[hi] == 1) s_fy[n][hst[hi]] = F;
-          if (hkind[hi] == 2) s_fz[n][hst[hi]] = F;
+          const T F = flux3(hc[W][hi][0], gx, hc[W][hi][1], gy, hc[W][hi][2], gz);
+          s_fl[n][hst[hi]] = F;
         }
       }
-      fetch_flux_geom(s + 1);
+      fetch_flux_geom(s + PD, W);
       __syncthreads();
 
       // Phase C: y/z divergence of plane s, then output plane o = s - 2.
@@ -373,27 +384,27 @@ __global__ __launch_bounds__(IG_THREADS) void monitor_stencil_kernel(MonParams<T
           const int ty = tyb + rr * RG;
           const int fy0 = (ty + HF) * TZ + tz;
           const int fz0 = ty * (TZ + 2 * HF) + tz + HF;
-          dyr[n][U][rr] = dop<CL, TR, T>(j0 + ty, ny, [&](int dd) { return s_fy[n][fy0 + dd * TZ]; }) +
-                          dop<CL, TR, T>(k0 + tz, nz, [&](int dd) { return s_fz[n][fz0 + dd]; });
+          dyr[n][U][rr] = dop<CL, TR, T>(j0 + ty, ny, [&](int dd) { return s_fl[n][fy0 + dd * TZ]; }) +
+                          dop<CL, TR, T>(k0 + tz, nz, [&](int dd) { return s_fl[n][FYS + fz0 + dd]; });
           const T dx = dop<CL, TR, T>(oc, nx, [&](int dd) { return fxr[n][(U - 2 + dd + NSLOT) % NSLOT][rr]; });
           const T lap = dx + dyr[n][(U - 2 + NSLOT) % NSLOT][rr];
           T v;
           if (TR) {
-            v = fma(vc[rr], xcur[n][rr], sc * lap);
+            v = fma(vc[W][rr], xcur[W][n][rr], sc * lap);
           } else {
-            const T psi = s_in[n][U * PLANE + (ty + HP) * PZ + tz + HP];
-            v = fma(vc[rr], psi, sc * (lap * ic[rr]));
+            const T psi = s_in[n][U * PSTR + (ty + HP) * PZ + tz + HP];
+            v = fma(vc[W][rr], psi, sc * (lap * ic[W][rr]));
           }
           if (p.interior_only && (obnd || ((tbnd >> rr) & 1u))) v = T(0);
           bst(rout[n], obn + toff[rr], v);
         }
       }
-      fetch_out_geom(s - 1);
+      fetch_out_geom(s - 2 + PD, W);
     }
   }
 }
 
-template <typename T, int TY, int TZ, int CL, bool TR, int NB = 1>
+template <typename T, int TY, int TZ, int CL, bool TR, int NB = 1, int PD = 1>
 int launch_monitor_t(const MonParams<T>& base, hipStream_t stream) {
   MonParams<T> p = base;
   // buffer-resource addressing: every field must stay below 1 GiB (the BAD offset)
@@ -401,60 +412,110 @@ int launch_monitor_t(const MonParams<T>& base, hipStream_t stream) {
   p.nty = (p.ny + TY - 1) / TY;
   p.ntz = (p.nz + TZ - 1) / TZ;
   if (p.xc <= 0) {
-    // Auto: split x so that at least ~2048 workgroups are in flight (8 per CU on a
-    // 256-CU MI355X), but never below 16 planes per chunk (each chunk re-reads 8 planes).
-    const long long base = (long long)p.nty * p.ntz * ((p.nblk + NB - 1) / NB);
-    long long nxc = (2048 + base - 1) / base;
-    const long long maxc = p.nx / 16 > 1 ? p.nx / 16 : 1;
-    if (nxc > maxc) nxc = maxc;
-    if (nxc < 1) nxc = 1;
-    p.xc = (int)((p.nx + nxc - 1) / nxc);
+    // Auto x chunking.  Blocks are latency bound, so the launch should fill every CU's
+    // resident-block capacity once, while each chunk pays ~5 extra plane steps (4 halo
+    // planes + the prologue).  Model time = rounds * (xc + 5) with rounds =
+    // ceil(blocks / capacity) and take the best chunk count.
+    static int occ = -1;
+    if (occ < 0) {
+      int n = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, monitor_stencil_kernel<T, TY, TZ, CL, TR, NB, PD>,
+                                                      IG_THREADS, 0) != hipSuccess || n < 1)
+        n = 1;
+      occ = n;
+    }
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+      cus = 256;
+    const long long capacity = (long long)occ * cus;
+    const long long tiles = (long long)p.nty * p.ntz;
+    const long long groups = (p.nblk + NB - 1) / NB;
+    const int maxc = p.nx / 8 > 1 ? p.nx / 8 : 1;
+    long long best = -1;
+    int best_c = 1;
+    for (int c = 1; c <= maxc; ++c) {
+      const int xc = (p.nx + c - 1) / c;
+      const long long blocks = 8LL * ((tiles * c + 7) / 8) * groups;
+      const long long rounds = (blocks + capacity - 1) / capacity;
+      const long long cost = rounds * (xc + 5);
+      if (best < 0 || cost < best) { best = cost; best_c = c; }
+    }
+    p.xc = (p.nx + best_c - 1) / best_c;
   }
   if (p.xc > p.nx) p.xc = p.nx;
   p.nxc = (p.nx + p.xc - 1) / p.xc;
   p.units = p.nty * p.ntz * p.nxc;
   const long long blocks = 8LL * ((p.nblk + NB - 1) / NB) * ((p.units + 7) / 8);
   if (blocks <= 0 || blocks > 0x7fffffffLL) return IG_EINVAL;
-  hipLaunchKernelGGL((monitor_stencil_kernel<T, TY, TZ, CL, TR, NB>), dim3((unsigned)blocks),
+  hipLaunchKernelGGL((monitor_stencil_kernel<T, TY, TZ, CL, TR, NB, PD>), dim3((unsigned)blocks),
                      dim3(IG_THREADS), 0, stream, p);
   IG_CHECK_LAUNCH();
   return IG_OK;
 }
 
-#define IG_MON_CASE(TY_, TZ_, NB_)                                                                 \
-  if (ty == TY_ && tz == TZ_ && nb == NB_)                                                         \
-    return closure ? launch_monitor_t<T, TY_, TZ_, 1, false, NB_>(p, s)                            \
-                   : launch_monitor_t<T, TY_, TZ_, 0, false, NB_>(p, s);
+#define IG_MON_CASE(TY_, TZ_, NB_, PD_)                                                            \
+  if (ty == TY_ && tz == TZ_ && nb == NB_ && pd == PD_)                                            \
+    return closure ? launch_monitor_t<T, TY_, TZ_, 1, false, NB_, PD_>(p, s)                       \
+                   : launch_monitor_t<T, TY_, TZ_, 0, false, NB_, PD_>(p, s);
 
 // tile: 0 = automatic; 32 / 64 = z tile width with the default rows; otherwise
-// NB*10000 + TZ*100 + TY (e.g. 23216: two orbitals per workgroup, 16 x 32 tiles).
+// PD*100000 + NB*10000 + TZ*100 + TY (e.g. 223216: two orbitals per workgroup, 16 x 32
+// tiles, loads issued two planes ahead).
 template <typename T>
 int launch_monitor(const MonParams<T>& p, int closure, int transpose, int tile, hipStream_t s) {
   if (transpose) return closure ? launch_monitor_t<T, 8, 32, 1, true>(p, s) : launch_monitor_t<T, 8, 32, 0, true>(p, s);
-  int nb = tile >= 10000 ? tile / 10000 : 0;
+  if (tile < 0) return IG_EINVAL;
+  int pd = tile / 100000;
+  int nb = (tile / 10000) % 10;
   const int t2 = tile % 10000;
   int tz = t2 >= 100 ? t2 / 100 : t2;
   int ty = t2 >= 100 ? t2 % 100 : 0;
+  if (tile == 0) {
+    // measured defaults (96^3 and 224^3 blocks): FP32 four orbitals of 8 x 32 tiles,
+    // FP64 two orbitals of 8 x 32 tiles with loads two planes ahead
+    tz = 32;
+    ty = 8;
+    if (sizeof(T) == 8) {
+      nb = p.nblk >= 2 ? 2 : 1;
+      pd = nb == 2 ? 2 : 1;
+    } else {
+      nb = p.nblk >= 4 ? 4 : (p.nblk >= 2 ? 2 : 1);
+      pd = 1;
+    }
+  }
   if (tz == 0) tz = 32;
-  if (ty == 0) ty = (sizeof(T) == 8) ? (tz == 64 ? 8 : 16) : 16;
+  if (ty == 0) ty = tz == 64 ? 8 : 16;
   if (nb == 0) nb = (p.nblk >= 2 && tz == 32) ? 2 : 1;
   if (nb > p.nblk) nb = (p.nblk >= 2 && tz == 32) ? 2 : 1;
+  if (pd == 0) pd = 1;
   if constexpr (sizeof(T) == 8) {
-    IG_MON_CASE(8, 64, 1)
-    IG_MON_CASE(8, 32, 1)
-    IG_MON_CASE(16, 32, 1)
-    IG_MON_CASE(8, 32, 2)
-    IG_MON_CASE(16, 32, 2)
+    IG_MON_CASE(8, 64, 1, 1)
+    IG_MON_CASE(8, 32, 1, 1)
+    IG_MON_CASE(8, 32, 1, 2)
+    IG_MON_CASE(16, 32, 1, 1)
+    IG_MON_CASE(16, 32, 1, 2)
+    IG_MON_CASE(8, 32, 2, 1)
+    IG_MON_CASE(8, 32, 2, 2)
+    IG_MON_CASE(8, 32, 2, 3)
+    IG_MON_CASE(16, 32, 2, 1)
+    IG_MON_CASE(16, 32, 2, 2)
   } else {
-    IG_MON_CASE(8, 64, 1)
-    IG_MON_CASE(16, 64, 1)
-    IG_MON_CASE(8, 32, 1)
-    IG_MON_CASE(16, 32, 1)
-    IG_MON_CASE(32, 32, 1)
-    IG_MON_CASE(8, 32, 2)
-    IG_MON_CASE(16, 32, 2)
-    IG_MON_CASE(8, 32, 4)
-    IG_MON_CASE(16, 32, 4)
+    IG_MON_CASE(8, 64, 1, 1)
+    IG_MON_CASE(8, 64, 1, 2)
+    IG_MON_CASE(8, 32, 1, 1)
+    IG_MON_CASE(16, 32, 1, 1)
+    IG_MON_CASE(16, 32, 1, 2)
+    IG_MON_CASE(16, 32, 1, 3)
+    IG_MON_CASE(8, 32, 2, 1)
+    IG_MON_CASE(8, 32, 2, 2)
+    IG_MON_CASE(8, 32, 2, 3)
+    IG_MON_CASE(16, 32, 2, 1)
+    IG_MON_CASE(16, 32, 2, 2)
+    IG_MON_CASE(16, 32, 2, 3)
+    IG_MON_CASE(8, 32, 4, 1)
+    IG_MON_CASE(8, 32, 4, 2)
+    IG_MON_CASE(16, 32, 4, 1)
   }
   return IG_EUNSUPPORTED;
 }
