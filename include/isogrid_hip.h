@@ -191,7 +191,8 @@ int ig_multipole_moments_f64(const double* rho, const double* w, const double* f
 int ig_boundary_values_f64(const double* xyz, int nx, int ny, int nz, const double* center,
                            int order, const double* qcoef, double* out, hipStream_t stream);
 /* Per selected cell (cells = ncells x (i, j, k), device int32): quadratic least-squares fit
- * of rho on the 4x4x4 node stencil -> coef[13] per cell.
+ * of rho on the 4x4x4 node stencil -> coef[14] per cell (10 coefficients in centred,
+ * scaled coordinates, the centroid and the inverse scale).
  * Replaces: poisson/open_boundary.py:299-342 (_cell_and_neighbor_slices,
  * _local_quadratic_fit_coefficients). */
 int ig_cell_fit_f64(const double* rho, const double* xyz, int nx, int ny, int nz, const int* cells,

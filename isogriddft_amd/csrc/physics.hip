@@ -380,17 +380,29 @@ __global__ __launch_bounds__(IG_THREADS) void boundary_kernel(const double* __re
 }
 
 // Quadratic least-squares fit of rho on the 4x4x4 node stencil of each selected cell,
-// in coordinates centred on the cell's first corner and scaled by the stencil extent.
-// Writes coef[cell][10] (basis 1, x, y, z, xx, xy, xz, yy, yz, zz of the scaled
-// coordinates), plus the centre and scale used.
+// in coordinates centred on the stencil centroid and scaled by the largest offset (the
+// fitted function does not depend on that affine change of basis; centring keeps the
+// 10 x 10 normal equations well conditioned).  Writes coef[cell][14]: the 10 basis
+// coefficients (1, x, y, z, xx, xy, xz, yy, yz, zz of the scaled coordinates), the
+// centroid and the inverse scale.
 __global__ __launch_bounds__(IG_THREADS) void cell_fit_kernel(const double* __restrict__ rho, const double* __restrict__ xyz,
                                                               int nx, int ny, int nz, const int* __restrict__ cells,
                                                               int ncells, double* __restrict__ coef) {
   const long long N = (long long)nx * ny * nz;
   for (int c = blockIdx.x * IG_THREADS + threadIdx.x; c < ncells; c += gridDim.x * IG_THREADS) {
     const int ci = cells[3 * c], cj = cells[3 * c + 1], ck = cells[3 * c + 2];
-    const long long g0 = ((long long)ci * ny + cj) * nz + ck;
-    const double ox = xyz[g0], oy = xyz[N + g0], oz = xyz[2 * N + g0];
+    double ox = 0.0, oy = 0.0, oz = 0.0;
+    for (int a = -1; a <= 2; ++a)
+      for (int b = -1; b <= 2; ++b)
+        for (int d = -1; d <= 2; ++d) {
+          const long long g = ((long long)(ci + a) * ny + (cj + b)) * nz + (ck + d);
+          ox += xyz[g];
+          oy += xyz[N + g];
+          oz += xyz[2 * N + g];
+        }
+    ox *= 1.0 / 64.0;
+    oy *= 1.0 / 64.0;
+    oz *= 1.0 / 64.0;
     double sc = 0.0;
     for (int a = -1; a <= 2; ++a)
       for (int b = -1; b <= 2; ++b)
@@ -436,11 +448,11 @@ __global__ __launch_bounds__(IG_THREADS) void cell_fit_kernel(const double* __re
       for (int s = r + 1; s < 10; ++s) v -= A[s * (s + 1) / 2 + r] * y[s];
       y[r] = v / A[r * (r + 1) / 2 + r];
     }
-    for (int r = 0; r < 10; ++r) coef[(long long)c * 13 + r] = y[r];
-    coef[(long long)c * 13 + 10] = ox;
-    coef[(long long)c * 13 + 11] = oy;
-    coef[(long long)c * 13 + 12] = oz;
-    (void)is;
+    for (int r = 0; r < 10; ++r) coef[(long long)c * 14 + r] = y[r];
+    coef[(long long)c * 14 + 10] = ox;
+    coef[(long long)c * 14 + 11] = oy;
+    coef[(long long)c * 14 + 12] = oz;
+    coef[(long long)c * 14 + 13] = is;
   }
 }
 
@@ -470,19 +482,8 @@ __global__ __launch_bounds__(IG_THREADS) void cell_moments_kernel(const double* 
           Z[e] = xyz[2 * N + g];
           Jc[e] = jac[g];
         }
-    const double* cf = coef + (long long)c * 13;
-    // scale of the fit basis: recompute the stencil extent exactly as the fit kernel
-    const long long g0 = ((long long)ci * ny + cj) * nz + ck;
-    const double ox = cf[10], oy = cf[11], oz = cf[12];
-    double sc = 0.0;
-    for (int a = -1; a <= 2; ++a)
-      for (int b = -1; b <= 2; ++b)
-        for (int d = -1; d <= 2; ++d) {
-          const long long g = ((long long)(ci + a) * ny + (cj + b)) * nz + (ck + d);
-          sc = fmax(sc, fmax(fabs(xyz[g] - ox), fmax(fabs(xyz[N + g] - oy), fabs(xyz[2 * N + g] - oz))));
-        }
-    (void)g0;
-    const double is = 1.0 / sc;
+    const double* cf = coef + (long long)c * 14;
+    const double ox = cf[10], oy = cf[11], oz = cf[12], is = cf[13];
     for (int s = 0; s < 8; ++s) {
       const double u = (s & 4) ? 0.75 : 0.25, v = (s & 2) ? 0.75 : 0.25, ww = (s & 1) ? 0.75 : 0.25;
       double tri[8];

@@ -1,0 +1,174 @@
+"""GPU parity of the pointwise physics kernels against the FP64 oracle.
+
+Bars: V_loc, its nuclear gradient, projector fields, the nonlocal block action and
+the LSDA potentials 1e-12 relative (max-abs over max|ref|); projector-position
+derivatives against central differences of the oracle fields 1e-6 relative.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from isogriddft_amd import configs, pseudo, xc
+from isogriddft_amd.device import DeviceGrid
+from oracle import grid as ogrid
+from oracle import pseudo as opseudo
+from oracle import xc as oxc
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / np.max(np.abs(b)))
+
+
+def _cases():
+    mol = ogrid.Case("CNO", (ogrid.Atom("C", (0.3, -0.2, 0.4)), ogrid.Atom("N", (-1.1, 0.5, -0.3)),
+                             ogrid.Atom("O", (0.9, 1.2, -1.0)), ogrid.Atom("H", (0.0, -1.4, 1.1))))
+    yield "synthetic30x34x38", configs.build_synthetic_geometry(shape=(30, 34, 38)), mol
+    yield "legacy25x27x29", ogrid.build_legacy_geometry(shape=(25, 27, 29), half_extents=(6.0, 6.0, 7.0),
+                                                        stretch=(2.5, 2.5, 2.8)), ogrid.h2o_case()
+
+
+def test_local_potential_matches_oracle():
+    for name, geom, case in _cases():
+        dg = DeviceGrid(geom)
+        v = pseudo.local_potential_device(dg, pseudo.case_atoms(case)).cpu().numpy().reshape(dg.shape)
+        ref = opseudo.local_ionic_potential(case, geom)
+        assert _rel(v, ref) <= TOL, (name, _rel(v, ref))
+        ev = pseudo.evaluate_local_ionic_potential(case, geom)
+        assert _rel(ev.total_local_potential, ref) <= TOL
+        assert len(ev.atom_contributions) == len(case.atoms)
+
+
+def test_local_potential_on_atom_site_limit():
+    """r = 0 takes the limit -Z sqrt(2/pi)/r_loc + C1 (pseudo/local.py:146-151)."""
+    geom = configs.build_synthetic_geometry(shape=(9, 9, 9))
+    dg = DeviceGrid(geom)
+    p = (float(geom.x_points[4, 4, 4]), float(geom.y_points[4, 4, 4]), float(geom.z_points[4, 4, 4]))
+    v = pseudo.local_potential_device(dg, [("O", p)]).cpu().numpy().reshape(dg.shape)
+    d = opseudo.GTH_PADE["O"]
+    assert abs(v[4, 4, 4] - (-d.zion * np.sqrt(2 / np.pi) / d.rloc + d.coefficients[0])) < 1e-12
+
+
+def test_local_potential_gradient_matches_oracle():
+    for name, geom, case in _cases():
+        dg = DeviceGrid(geom)
+        atoms = pseudo.case_atoms(case)
+        rng = np.random.default_rng(1)
+        rho = np.exp(-0.3 * (geom.x_points ** 2 + geom.y_points ** 2 + geom.z_points ** 2)) * (1 + 0.1 * rng.random(dg.shape))
+        q = rho * geom.cell_volumes
+        g = pseudo.local_potential_gradient_device(dg, atoms, torch.from_numpy(q).cuda())
+        ref = np.zeros_like(g)
+        for a, atom in enumerate(case.atoms):
+            d = opseudo.GTH_PADE[atom.element]
+            dx = geom.x_points - atom.position[0]
+            dy = geom.y_points - atom.position[1]
+            dz = geom.z_points - atom.position[2]
+            r = np.sqrt(dx * dx + dy * dy + dz * dz)
+            dv = opseudo.local_potential_radial_derivative(r, d)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                f = np.where(r > 0, q * dv / r, 0.0)
+            ref[a] = [-np.sum(f * dx), -np.sum(f * dy), -np.sum(f * dz)]
+        assert np.max(np.abs(g - ref)) <= TOL * np.max(np.abs(ref)), (name, g, ref)
+
+
+def _orbitals(geom, k, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(k):
+        c = rng.uniform(-1.5, 1.5, 3)
+        a = rng.uniform(0.3, 1.5)
+        out.append(np.exp(-a * ((geom.x_points - c[0]) ** 2 + (geom.y_points - c[1]) ** 2 + (geom.z_points - c[2]) ** 2))
+                   * (1 + 0.3 * np.sin(geom.x_points + 2 * geom.y_points)))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_nonlocal_block_action_matches_oracle(dtype):
+    for name, geom, case in _cases():
+        dg = DeviceGrid(geom)
+        psi = _orbitals(geom, 3, 7)
+        plan = pseudo.NonlocalPlan(dg, case)
+        x = torch.from_numpy(psi).to("cuda", dtype)
+        out = torch.zeros_like(x)
+        plan.apply(x, out)
+        got = out.double().cpu().numpy()
+        ref = np.array([opseudo.nonlocal_action(p, case, geom) for p in psi])
+        err = _rel(got, ref)
+        tol = TOL if dtype == torch.float64 else 5e-6
+        assert err <= tol, (name, dtype, err)
+        # E_nl pieces: <psi|V_nl psi>_W
+        e = plan.energy_terms(x)
+        e_ref = np.array([np.sum(p * r * geom.cell_volumes) for p, r in zip(psi, ref)])
+        assert np.max(np.abs(e - e_ref)) <= (1e-12 if dtype == torch.float64 else 1e-5) * np.max(np.abs(e_ref))
+
+
+def test_projector_fields_and_position_derivatives():
+    name, geom, case = next(_cases())
+    dg = DeviceGrid(geom)
+    plan = pseudo.NonlocalPlan(dg, case)
+    fields = {ia: (f, H) for ia, f, H in opseudo.projector_fields(case, geom)}
+    h = 1e-5
+    for ap in plan.per_atom:
+        b = list(ap.box)
+        sl = (slice(b[0], b[0] + b[3]), slice(b[1], b[1] + b[4]), slice(b[2], b[2] + b[5]))
+        ref = fields[ap.atom_index][0]
+        got = ap.P.cpu().numpy().reshape(ap.nproj, b[3], b[4], b[5])
+        assert _rel(got, ref[(slice(None),) + sl]) <= TOL
+        # outside the box the reference field is negligible
+        mask = np.ones(dg.shape, dtype=bool)
+        mask[sl] = False
+        assert np.max(np.abs(ref[:, mask])) <= 1e-18 * np.max(np.abs(ref))
+        for axis in range(3):
+            dP = ap.field_derivative(dg, axis).cpu().numpy().reshape(ap.nproj, b[3], b[4], b[5])
+            atoms_p = list(case.atoms)
+            pos = list(atoms_p[ap.atom_index].position)
+            pp, pm = list(pos), list(pos)
+            pp[axis] += h
+            pm[axis] -= h
+            cp = ogrid.Case("p", tuple(ogrid.Atom(a.element, tuple(pp) if n == ap.atom_index else a.position)
+                                       for n, a in enumerate(atoms_p)))
+            cm = ogrid.Case("m", tuple(ogrid.Atom(a.element, tuple(pm) if n == ap.atom_index else a.position)
+                                       for n, a in enumerate(atoms_p)))
+            fp = {ia: f for ia, f, _ in opseudo.projector_fields(cp, geom)}[ap.atom_index]
+            fm = {ia: f for ia, f, _ in opseudo.projector_fields(cm, geom)}[ap.atom_index]
+            fd = (fp - fm)[(slice(None),) + sl] / (2 * h)
+            assert _rel(dP, fd) <= 1e-6, (ap.element, axis, _rel(dP, fd))
+
+
+def test_nonlocal_call_surface():
+    name, geom, case = next(_cases())
+    psi = _orbitals(geom, 1, 3)[0]
+    ev = pseudo.evaluate_nonlocal_ionic_action(case, geom, psi)
+    ref = opseudo.nonlocal_action(psi, case, geom)
+    assert _rel(ev.total_nonlocal_action, ref) <= TOL
+    assert len(ev.atom_contributions) == 3  # C, N, O carry projectors; H has none
+    s = sum(c.nonlocal_action for c in ev.atom_contributions)
+    assert _rel(s, ref) <= TOL
+
+
+def test_lsda_matches_oracle_and_rejects_negative_density():
+    rng = np.random.default_rng(11)
+    shape = (20, 21, 22)
+    up = rng.uniform(0, 2, shape) ** 3
+    dn = rng.uniform(0, 1, shape) ** 3
+    up[0, 0, :4] = 0.0
+    dn[1, 1, :4] = 0.0
+    up[2, 2, 0] = dn[2, 2, 0] = 0.0
+    up[3, 3, 0] = -5e-15
+    ev = xc.evaluate_lsda_terms(up, dn)
+    eps, vu, vd = oxc.lsda_pw92(up, dn)
+    assert _rel(ev.eps_xc, eps) <= TOL
+    assert _rel(ev.v_xc_up, vu) <= TOL
+    assert _rel(ev.v_xc_down, vd) <= TOL
+    with pytest.raises(ValueError):
+        xc.evaluate_lsda_terms(up - 1e-3, dn)
+    with pytest.raises(ValueError):
+        xc.evaluate_lsda_terms(up, dn, functional="lda,vwn")
+    bad = dn.copy()
+    bad[0, 0, 0] = np.nan
+    with pytest.raises(ValueError):
+        xc.evaluate_lsda_terms(up, bad)
