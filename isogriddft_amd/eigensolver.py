@@ -1,0 +1,573 @@
+"""Fixed-potential block eigensolver on the GPU (drop-in for the reference's
+ks/eigensolver.py entry points and the block iteration of ks/eigensolver_jax.py).
+
+Algorithm (ks/eigensolver_jax.py:417-524, restated on device-resident blocks):
+  X <- weighted QR(X0);  HX <- H X
+  repeat:
+    P = X^T W HX (symmetrised), (theta, V) = eigh(P); C = V[:, :k]
+    Psi = C^T X, HPsi = C^T HX, R = HPsi - theta Psi, r = ||R||_W
+    stop when max r <= tol
+    R <- R - (R^T W X) X, whitened by (R^T W R)^(-1/2)     (ks/eigensolver_jax.py:119-146)
+    HR = H R
+    T = [[P, X^T W HR], [R^T W HX, R^T W HR]], lowest m Ritz vectors [c1; c2]
+    X <- c1^T X + c2^T R, HX <- c1^T HX + c2^T HR, both whitened by (X^T W X)^(-1/2)
+Device work per iteration: one H apply of k orbitals (the fused HIP stencil), the
+weighted Gram products X^T W Y on MFMA (FP32 blocks) or FP64 FMAs (FP64 blocks) with
+FP64 accumulation (ig_gram_*), the basis rotations as library GEMMs (torch.matmul ->
+hipBLASLt), and the small FP64 eigendecompositions (torch.linalg.eigh on the GPU).
+
+Precision: "fp64" runs every step in FP64.  "mixed" iterates with FP32 blocks (FP32
+stencil, MFMA Gram) until the residuals reach max(tol, fp32 floor), then promotes the
+basis to FP64, re-applies H in FP64 and finishes with FP64 iterations (refinement).
+"""
+
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import configs
+from .device import DeviceGrid, device_grid
+from .hamiltonian import (BlockHamiltonian, H2_BENCHMARK_CASE, normalize_kinetic_version, normalize_spin_channel,
+                          resolve_hartree, resolve_xc)
+from .reductions import gram
+
+FP32_RESIDUAL_FLOOR = 5e-5
+
+
+# ------------------------------------------------------------------ weighted block algebra
+
+
+def _sym(P):
+    return 0.5 * (P + P.T)
+
+
+def _whiten(C, w, floor=1e-10):
+    """(T^T C, T) with T = (C^T W C)^(-1/2) (eigenvalue floor 1e-10), else a weighted
+    Cholesky-QR transform (ks/eigensolver_jax.py:119-146)."""
+    G = _sym(gram(C, C, w))
+    ev, V = torch.linalg.eigh(G)
+    if bool(torch.isfinite(ev).all()) and float(ev.min()) > floor:
+        T = (V * (1.0 / torch.sqrt(ev.clamp_min(1e-14)))[None, :]) @ V.T
+    else:
+        L = torch.linalg.cholesky(G + 1e-14 * float(ev.abs().max()) * torch.eye(G.shape[0], dtype=G.dtype,
+                                                                                  device=G.device))
+        T = torch.linalg.inv(L).T  # C^T W C = L L^T  ->  (L^-1 C) is W-orthonormal
+    out = torch.matmul(T.T.to(C.dtype), C.reshape(C.shape[0], -1))
+    return out, T
+
+
+def weighted_qr(X, w):
+    """Rows of X made W-orthonormal by R^(-1) of the weighted QR (Cholesky-QR on the
+    FP64 Gram, ks/eigensolver_jax.py:108-117)."""
+    G = _sym(gram(X, X, w))
+    L = torch.linalg.cholesky(G)
+    Linv = torch.linalg.inv(L)
+    return torch.matmul(Linv.to(X.dtype), X.reshape(X.shape[0], -1))
+
+
+def orthonormalize_device(X, w, rank_tolerance=1e-12, require_full_rank=True):
+    """SVD-equivalent weighted orthonormalisation (ks/eigensolver.py:292-341) from the
+    FP64 Gram X^T W X = V S^2 V^T: columns X V S^-1 ordered by decreasing S."""
+    m = X.shape[0]
+    N = X.reshape(m, -1).shape[1]
+    G = _sym(gram(X, X, w))
+    ev, V = torch.linalg.eigh(G)
+    ev = torch.flip(ev, [0])
+    V = torch.flip(V, [1])
+    s = torch.sqrt(ev.clamp_min(0.0))
+    if m == 0 or float(s[0]) == 0.0:
+        if require_full_rank:
+            raise ValueError("No nonzero orbital directions were provided for orthonormalization.")
+        return X.new_zeros((0,) + tuple(X.shape[1:]))
+    cutoff = rank_tolerance * max(N, m) * float(s[0])
+    rank = int((s > cutoff).sum())
+    if require_full_rank and rank < m:
+        raise ValueError("The orbital block is rank-deficient under weighted orthonormalization: "
+                         f"rank={rank}, requested={m}.")
+    T = V[:, :rank] / s[None, :rank]
+    out = torch.matmul(T.T.to(X.dtype), X.reshape(m, -1))
+    return out.reshape((rank,) + tuple(X.shape[1:]))
+
+
+@dataclass
+class BlockIterationResult:
+    eigenvalues: np.ndarray
+    orbitals: torch.Tensor
+    residual_norms: np.ndarray
+    iterations: int
+    converged: bool
+    residual_history: list = field(default_factory=list)
+    ritz_value_history: list = field(default_factory=list)
+    subspace_dimensions: list = field(default_factory=list)
+    ritz_matrix: np.ndarray | None = None
+    basis: torch.Tensor | None = None
+    fp32_iterations: int = 0
+    timings: dict = field(default_factory=dict)
+
+
+def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp32_floor=FP32_RESIDUAL_FLOOR,
+                             keep_basis=False):
+    """Lowest-k eigenpairs of the W-self-adjoint H from the starting block X0 (m, N)."""
+    if precision not in ("fp64", "mixed"):
+        raise ValueError(f"precision must be `fp64` or `mixed`; received `{precision}`.")
+    m = X0.shape[0]
+    if k > m:
+        raise ValueError("the starting block must hold at least k orbitals")
+    shape = H.dg.shape
+    N = H.dg.N
+    w64 = w.reshape(-1).to(torch.float64)
+    dt = torch.float32 if precision == "mixed" else torch.float64
+    wdt = w64.to(dt)
+    tm = {"hamiltonian_apply": 0.0, "gram": 0.0, "eigh": 0.0, "rotation": 0.0}
+
+    def sync_t():
+        torch.cuda.synchronize()
+        return time.perf_counter()
+
+    def apply(B):
+        return H.apply(B.view((B.shape[0],) + shape)).reshape(B.shape[0], N)
+
+    X = weighted_qr(X0.reshape(m, N).to(dt), wdt)
+    HX = apply(X)
+    res = BlockIterationResult(np.zeros(k), X[:k], np.full(k, np.inf), 0, False)
+    phase32 = dt == torch.float32
+    it = 0
+    for it in range(1, max_iter + 1):
+        P = _sym(gram(X, HX, wdt))
+        theta, V = torch.linalg.eigh(P)
+        C = V[:, :k]
+        Ct = C.T.to(dt)
+        Psi = torch.matmul(Ct, X)
+        HPsi = torch.matmul(Ct, HX)
+        R = HPsi - theta[:k, None].to(dt) * Psi
+        rn = torch.sqrt(torch.diagonal(gram(R, R, wdt)).clamp_min(0.0)).cpu().numpy()
+        th = theta[:k].cpu().numpy()
+        res.residual_history.append(rn)
+        res.ritz_value_history.append(th)
+        res.subspace_dimensions.append(m)
+        target = max(tol, fp32_floor) if phase32 else tol
+        stalled = phase32 and len(res.residual_history) > 6 and \
+            rn.max() >= 0.9 * min(r.max() for r in res.residual_history[-6:-1])
+        if rn.max() <= target or stalled:
+            if phase32:
+                # refinement: promote the basis to FP64 and rebuild H X in FP64
+                res.fp32_iterations = it
+                phase32 = False
+                dt = torch.float64
+                wdt = w64
+                X = weighted_qr(X.double(), wdt)
+                HX = apply(X)
+                continue
+            res.eigenvalues, res.orbitals, res.residual_norms = th, Psi, rn
+            res.iterations, res.converged = it, True
+            res.ritz_matrix = P[:k, :k].cpu().numpy()
+            break
+        R = R - torch.matmul(gram(R, X, wdt).to(dt), X)
+        R, _ = _whiten(R, wdt)
+        HR = apply(R)
+        T = torch.cat([torch.cat([P, gram(X, HR, wdt)], 1), torch.cat([gram(R, HX, wdt), gram(R, HR, wdt)], 1)], 0)
+        _, TV = torch.linalg.eigh(_sym(T))
+        c1t = TV[:m, :m].T.to(dt).contiguous()
+        c2t = TV[m:, :m].T.to(dt).contiguous()
+        Xn = torch.addmm(torch.matmul(c2t, R), c1t, X)
+        HXn = torch.addmm(torch.matmul(c2t, HR), c1t, HX)
+        X, Tr = _whiten(Xn, wdt)
+        HX = torch.matmul(Tr.T.to(dt), HXn)
+        res.eigenvalues, res.orbitals, res.residual_norms = th, Psi, rn
+        res.iterations = it
+        res.ritz_matrix = P[:k, :k].cpu().numpy()
+    if keep_basis:
+        res.basis = X
+    res.orbitals = res.orbitals.reshape((k,) + shape)
+    return res
+
+
+# ------------------------------------------------------------------ default guesses
+
+
+def default_guess_block(grid_geometry, atoms, count, dg=None):
+    """Deterministic + rng(12345) guesses, W-orthonormalised (ks/eigensolver.py:818-879)."""
+    from .grid import grid_kind, reference_center
+
+    dg = dg or device_grid(grid_geometry)
+    if grid_kind(grid_geometry) == "monitor":
+        bb = dg.box_bounds
+        center = tuple(0.5 * (b[0] + b[1]) for b in bb)
+    else:
+        center = tuple(reference_center(grid_geometry))
+    xyz = dg.xyz()
+    x, y, z = (xyz[i] - center[i] for i in range(3))
+    r2 = x * x + y * y + z * z
+    guesses = []
+    gs = []
+    for _, pos in atoms:
+        d2 = (xyz[0] - pos[0]) ** 2 + (xyz[1] - pos[1]) ** 2 + (xyz[2] - pos[2]) ** 2
+        gs.append(torch.exp(-0.85 * d2))
+    if gs:
+        guesses.append(sum(gs))
+        if len(gs) >= 2:
+            guesses.append(sum(g if i % 2 == 0 else -g for i, g in enumerate(gs)))
+    base = torch.exp(-0.65 * r2)
+    guesses += [base, z * torch.exp(-0.55 * r2), x * torch.exp(-0.55 * r2), y * torch.exp(-0.55 * r2),
+                torch.exp(-1.10 * r2), (3.0 * z * z - r2) * torch.exp(-0.45 * r2)]
+    total = max(count, 8)
+    rng = np.random.default_rng(12345)
+    while len(guesses) < total:
+        f = torch.from_numpy(rng.normal(size=dg.shape).reshape(-1)).to(dg.device)
+        guesses.append(base * f)
+    block = torch.stack(guesses[:total])
+    ortho = orthonormalize_device(block, dg.weights(), require_full_rank=False)
+    if ortho.shape[0] < count:
+        raise ValueError("Unable to build enough independent default guess orbitals for the fixed-potential solve.")
+    return ortho[:count]
+
+
+def initial_guess_block(grid_geometry, atoms, k, basis_size, initial_guess_orbitals=None, dg=None):
+    """ks/eigensolver.py:882-913."""
+    dg = dg or device_grid(grid_geometry)
+    target = max(k, basis_size)
+    default = default_guess_block(grid_geometry, atoms, target, dg)
+    if initial_guess_orbitals is None:
+        return default[:target]
+    user = torch.as_tensor(np.asarray(initial_guess_orbitals, dtype=np.float64)).to(dg.device).reshape(-1, dg.N)
+    comb = torch.cat([user, default], 0)
+    ortho = orthonormalize_device(comb, dg.weights(), require_full_rank=False)
+    if ortho.shape[0] < k:
+        raise ValueError("The provided initial guesses do not span enough weighted-independent directions.")
+    return ortho[:target]
+
+
+def reference_block_size(k, initial_subspace_size=None, ncv=None):
+    """Working-subspace size of ks/eigensolver_jax.py:81-87."""
+    return max(2 * k + 2, 4, 12 * k + 8, 0 if initial_subspace_size is None else int(initial_subspace_size),
+               0 if ncv is None else int(ncv))
+
+
+# ------------------------------------------------------------------ operator contexts
+
+
+@dataclass(frozen=True)
+class FixedPotentialStaticLocalOperatorContext:
+    """Fields of ks/eigensolver.py:80-106 (patch embedding: None, not built here)."""
+
+    grid_geometry: object
+    case: object
+    spin_channel: str
+    rho_up: np.ndarray
+    rho_down: np.ndarray
+    rho_total: np.ndarray
+    local_ionic_potential: np.ndarray
+    hartree_potential: np.ndarray
+    xc_potential: np.ndarray
+    effective_local_potential: np.ndarray
+    local_ionic_evaluation: object
+    frozen_patch_local_embedding: object
+    hartree_poisson_result: object
+    lsda_evaluation: object
+    kinetic_version: str = "production"
+
+
+@dataclass(frozen=True)
+class FixedPotentialOperatorContext:
+    """Fields of ks/eigensolver.py:60-77 (static KS: local + nonlocal + Hartree + XC)."""
+
+    grid_geometry: object
+    case: object
+    spin_channel: str
+    rho_up: np.ndarray
+    rho_down: np.ndarray
+    rho_total: np.ndarray
+    local_ionic_potential: np.ndarray
+    hartree_potential: np.ndarray
+    xc_potential: np.ndarray
+    effective_local_potential: np.ndarray
+    local_ionic_evaluation: object
+    hartree_poisson_result: object
+    lsda_evaluation: object
+
+
+@dataclass(frozen=True)
+class FixedPotentialStaticLocalPreparationProfile:
+    density_build_wall_time_seconds: float
+    local_ionic_resolve_wall_time_seconds: float
+    hartree_resolve_wall_time_seconds: float
+    xc_resolve_wall_time_seconds: float
+    total_wall_time_seconds: float
+
+
+def _resolve_local(grid_geometry, dg, case, local_ionic_potential):
+    from .hamiltonian import _resolve_local as rl
+
+    return rl(grid_geometry, dg, case, local_ionic_potential)
+
+
+def _density_pair(grid_geometry, dg, rho_up, rho_down):
+    from .hamiltonian import _field
+
+    up = _field(rho_up, dg, "rho_up")
+    dn = _field(rho_down, dg, "rho_down")
+    return up, dn, up + dn
+
+
+def prepare_fixed_potential_static_local_operator_profiled(
+        grid_geometry, rho_up, rho_down, spin_channel, case=H2_BENCHMARK_CASE, local_ionic_potential=None,
+        hartree_potential=None, xc_potential=None, xc_functional=None, *, use_monitor_patch=False,
+        patch_parameters=None, kinetic_version="production", base_local_ionic_evaluation=None,
+        hartree_backend="python", use_jax_hartree_cached_operator=False, jax_hartree_cg_impl="baseline",
+        jax_hartree_cg_preconditioner="none", jax_hartree_line_preconditioner_impl="baseline",
+        monitor_boundary_construction_mode="corrected_moments", hartree_precision="fp64"):
+    """ks/eigensolver.py:571-671: freeze T + V_loc + V_H + V_xc on one grid."""
+    if use_monitor_patch:
+        raise NotImplementedError("the near-core patch embedding is not built on the HIP path; pass the "
+                                  "patched local potential as `local_ionic_potential`.")
+    t0 = time.perf_counter()
+    dg = device_grid(grid_geometry)
+    spin = normalize_spin_channel(spin_channel)
+    kv = normalize_kinetic_version(kinetic_version)
+    up, dn, tot = _density_pair(grid_geometry, dg, rho_up, rho_down)
+    t1 = time.perf_counter()
+    local, local_ev = _resolve_local(grid_geometry, dg, case, local_ionic_potential)
+    t2 = time.perf_counter()
+    vh, _, hres = resolve_hartree(grid_geometry, dg, tot, hartree_potential, backend=hartree_backend,
+                                  monitor_boundary_construction_mode=monitor_boundary_construction_mode,
+                                  precision=hartree_precision)
+    t3 = time.perf_counter()
+    vxc, lsda = resolve_xc(dg, up, dn, spin, xc_potential, xc_functional)
+    t4 = time.perf_counter()
+    ctx = FixedPotentialStaticLocalOperatorContext(
+        grid_geometry, case, spin, up, dn, tot, np.asarray(local), np.asarray(vh), np.asarray(vxc),
+        np.asarray(local + vh + vxc, dtype=np.float64), local_ev, None, hres, lsda, kv)
+    prof = FixedPotentialStaticLocalPreparationProfile(t1 - t0, t2 - t1, t3 - t2, t4 - t3, time.perf_counter() - t0)
+    return ctx, prof
+
+
+def prepare_fixed_potential_static_local_operator(grid_geometry, rho_up, rho_down, spin_channel,
+                                                  case=H2_BENCHMARK_CASE, local_ionic_potential=None,
+                                                  hartree_potential=None, xc_potential=None, xc_functional=None,
+                                                  **kwargs):
+    """ks/eigensolver.py:674-726."""
+    return prepare_fixed_potential_static_local_operator_profiled(
+        grid_geometry, rho_up, rho_down, spin_channel, case, local_ionic_potential, hartree_potential,
+        xc_potential, xc_functional, **kwargs)[0]
+
+
+def prepare_fixed_potential_static_ks_operator(grid_geometry, rho_up, rho_down, spin_channel,
+                                               case=H2_BENCHMARK_CASE, local_ionic_potential=None,
+                                               hartree_potential=None, xc_potential=None, xc_functional=None):
+    """ks/eigensolver.py:477-531 (the nonlocal term is applied per block by the operator)."""
+    dg = device_grid(grid_geometry)
+    spin = normalize_spin_channel(spin_channel)
+    up, dn, tot = _density_pair(grid_geometry, dg, rho_up, rho_down)
+    local, local_ev = _resolve_local(grid_geometry, dg, case, local_ionic_potential)
+    vh, _, hres = resolve_hartree(grid_geometry, dg, tot, hartree_potential)
+    vxc, lsda = resolve_xc(dg, up, dn, spin, xc_potential, xc_functional)
+    return FixedPotentialOperatorContext(grid_geometry, case, spin, up, dn, tot, np.asarray(local), np.asarray(vh),
+                                         np.asarray(vxc), np.asarray(local + vh + vxc, dtype=np.float64), local_ev,
+                                         hres, lsda)
+
+
+def block_hamiltonian(operator_context) -> BlockHamiltonian:
+    """The device operator of a context (static-local: T + V_eff; static KS: + V_nl)."""
+    from . import pseudo
+
+    dg = device_grid(operator_context.grid_geometry)
+    cache = getattr(dg, "_hops", None)
+    if cache is None:
+        cache = {}
+        dg._hops = cache
+    key = id(operator_context)
+    hit = cache.get(key)
+    if hit is not None and hit[0] is operator_context:
+        return hit[1]
+    veff = torch.from_numpy(np.ascontiguousarray(operator_context.effective_local_potential)).to(dg.device)
+    if isinstance(operator_context, FixedPotentialOperatorContext):
+        op = BlockHamiltonian(dg, veff, closure="production",
+                              nonlocal_plan=pseudo.nonlocal_plan(dg, operator_context.case))
+    else:
+        op = BlockHamiltonian(dg, veff, closure=operator_context.kinetic_version)
+    cache.clear()
+    cache[key] = (operator_context, op)
+    return op
+
+
+def _apply_block_numpy(orbitals, operator_context):
+    dg = device_grid(operator_context.grid_geometry)
+    blk = np.asarray(orbitals, dtype=np.float64)
+    single = blk.shape == dg.shape
+    x = torch.from_numpy(np.ascontiguousarray(blk.reshape((-1,) + dg.shape))).to(dg.device)
+    out = block_hamiltonian(operator_context).apply(x).cpu().numpy()
+    return out[0] if single else out
+
+
+def apply_fixed_potential_static_local_operator(psi, operator_context):
+    """ks/eigensolver.py:729-756."""
+    return _apply_block_numpy(psi, operator_context)
+
+
+def apply_fixed_potential_static_local_block(orbitals, operator_context):
+    """ks/eigensolver.py:773-784 (one fused block launch instead of a per-orbital loop)."""
+    return _apply_block_numpy(orbitals, operator_context)
+
+
+def apply_fixed_potential_static_ks_operator(psi, operator_context):
+    """ks/eigensolver.py:534-556."""
+    return _apply_block_numpy(psi, operator_context)
+
+
+def apply_fixed_potential_static_ks_block(orbitals, operator_context):
+    """ks/eigensolver.py:804-815."""
+    return _apply_block_numpy(orbitals, operator_context)
+
+
+# ------------------------------------------------------------------ results and entry points
+
+
+@dataclass(frozen=True)
+class FixedPotentialEigensolverResult:
+    """Fields of ks/eigensolver.py:121-154."""
+
+    target_orbitals: int
+    solver_backend: str
+    use_scipy_fallback: bool
+    solver_method: str
+    solver_note: str
+    converged: bool
+    iteration_count: int
+    tolerance: float
+    eigenvalues: np.ndarray
+    orbitals: np.ndarray
+    weighted_overlap: np.ndarray
+    max_orthogonality_error: float
+    residual_norms: np.ndarray
+    residual_history: np.ndarray
+    ritz_value_history: np.ndarray
+    subspace_dimensions: tuple
+    ritz_matrix: np.ndarray
+    initial_guess_orbitals: np.ndarray
+    final_basis_orbitals: np.ndarray
+    operator_context: object
+    static_local_preparation_profile: object = None
+    use_jax_block_kernels: bool = False
+    use_jax_cached_kernels: bool = False
+    wall_time_seconds: float | None = None
+    jax_internal_profile: object = None
+    precision: str = "fp64"
+    fp32_iterations: int = 0
+
+    @property
+    def internal_profile(self):
+        return self.jax_internal_profile
+
+
+_BACKENDS = ("auto", "jax", "scipy_fallback", "hip")
+
+
+def _solve(operator_context, k, initial_guess_orbitals, max_iterations, tolerance, initial_subspace_size, ncv,
+           backend, precision, profile, keep_basis=True):
+    from . import pseudo
+
+    if k <= 0:
+        raise ValueError(f"k must be positive; received {k}.")
+    if max_iterations <= 0:
+        raise ValueError(f"max_iterations must be positive; received {max_iterations}.")
+    if tolerance <= 0.0:
+        raise ValueError(f"tolerance must be positive; received {tolerance}.")
+    nb = str(backend).strip().lower()
+    if nb not in _BACKENDS:
+        raise ValueError(f"solver_backend must be one of {_BACKENDS}; received `{backend}`.")
+    t0 = time.perf_counter()
+    geom = operator_context.grid_geometry
+    dg = device_grid(geom)
+    m = reference_block_size(k, initial_subspace_size, ncv)
+    X0 = initial_guess_block(geom, pseudo.case_atoms(operator_context.case), k, m, initial_guess_orbitals, dg)
+    m = min(m, X0.shape[0])
+    X0 = X0[:m]
+    H = block_hamiltonian(operator_context)
+    res = block_subspace_iteration(H, dg.weights(), X0, k, tolerance, max_iterations, precision=precision,
+                                   keep_basis=keep_basis)
+    orbs = res.orbitals.double()
+    S = gram(orbs, orbs, dg.weights()).cpu().numpy()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    c = lambda t: None if t is None else t.double().cpu().numpy().reshape((-1,) + dg.shape)  # noqa: E731
+    return FixedPotentialEigensolverResult(
+        target_orbitals=k, solver_backend="hip" if nb == "auto" else nb, use_scipy_fallback=False,
+        solver_method=f"hip_block_subspace_iteration_{precision}",
+        solver_note=("block subspace iteration with residual expansion and Rayleigh-Ritz restart on the GPU "
+                     f"(working subspace {m}, precision {precision})"),
+        converged=res.converged, iteration_count=res.iterations, tolerance=float(tolerance),
+        eigenvalues=np.asarray(res.eigenvalues, dtype=np.float64), orbitals=c(orbs), weighted_overlap=S,
+        max_orthogonality_error=float(np.max(np.abs(S - np.eye(k)))), residual_norms=np.asarray(res.residual_norms),
+        residual_history=np.asarray(res.residual_history), ritz_value_history=np.asarray(res.ritz_value_history),
+        subspace_dimensions=tuple(res.subspace_dimensions), ritz_matrix=res.ritz_matrix,
+        initial_guess_orbitals=c(X0), final_basis_orbitals=c(res.basis), operator_context=operator_context,
+        static_local_preparation_profile=profile, wall_time_seconds=wall, precision=precision,
+        fp32_iterations=res.fp32_iterations)
+
+
+def solve_fixed_potential_static_local_eigenproblem(
+        grid_geometry, rho_up, rho_down, spin_channel, k, case=H2_BENCHMARK_CASE, initial_guess_orbitals=None,
+        local_ionic_potential=None, hartree_potential=None, xc_potential=None, xc_functional=None,
+        operator_context=None, max_iterations=400, tolerance=1.0e-3, initial_subspace_size=None, ncv=None, *,
+        use_monitor_patch=False, patch_parameters=None, kinetic_version="production", use_jax_block_kernels=False,
+        operator_preparation_profile=None, base_local_ionic_evaluation=None, hartree_backend="python",
+        use_jax_hartree_cached_operator=False, jax_hartree_cg_impl="baseline", jax_hartree_cg_preconditioner="none",
+        jax_hartree_line_preconditioner_impl="baseline", monitor_boundary_construction_mode="corrected_moments",
+        solver_backend="auto", profile_jax_internals=False, precision="fp64"):
+    """ks/eigensolver.py:1183-1320 on the GPU (every backend value runs the block iteration)."""
+    if operator_context is None:
+        operator_context, operator_preparation_profile = prepare_fixed_potential_static_local_operator_profiled(
+            grid_geometry, rho_up, rho_down, spin_channel, case, local_ionic_potential, hartree_potential,
+            xc_potential, xc_functional, use_monitor_patch=use_monitor_patch, patch_parameters=patch_parameters,
+            kinetic_version=kinetic_version, base_local_ionic_evaluation=base_local_ionic_evaluation,
+            hartree_backend=hartree_backend, monitor_boundary_construction_mode=monitor_boundary_construction_mode)
+    return _solve(operator_context, k, initial_guess_orbitals, max_iterations, tolerance, initial_subspace_size,
+                  ncv, solver_backend, precision, operator_preparation_profile)
+
+
+def solve_fixed_potential_eigenproblem(grid_geometry, rho_up, rho_down, spin_channel, k, case=H2_BENCHMARK_CASE,
+                                       initial_guess_orbitals=None, local_ionic_potential=None,
+                                       hartree_potential=None, xc_potential=None, xc_functional=None,
+                                       max_iterations=400, tolerance=1.0e-3, initial_subspace_size=None, ncv=None,
+                                       *, precision="fp64"):
+    """ks/eigensolver.py:1136-1180: the static-KS operator (with nonlocal projectors)."""
+    ctx = prepare_fixed_potential_static_ks_operator(grid_geometry, rho_up, rho_down, spin_channel, case,
+                                                     local_ionic_potential, hartree_potential, xc_potential,
+                                                     xc_functional)
+    return _solve(ctx, k, initial_guess_orbitals, max_iterations, tolerance, initial_subspace_size, ncv, "auto",
+                  precision, None)
+
+
+# ------------------------------------------------------------------ overlaps (NumPy surfaces)
+
+
+def weighted_overlap_matrix(orbitals, grid_geometry, other=None, *, use_jax_block_kernels=False):
+    """ks/eigensolver.py:246-272 (X^T W Y with FP64 accumulation on the GPU)."""
+    dg = device_grid(grid_geometry)
+    a = torch.from_numpy(np.ascontiguousarray(np.asarray(orbitals, dtype=np.float64))).to(dg.device)
+    a = a.reshape(-1, dg.N)
+    b = a if other is None else torch.from_numpy(
+        np.ascontiguousarray(np.asarray(other, dtype=np.float64))).to(dg.device).reshape(-1, dg.N)
+    return gram(a, b, dg.weights()).cpu().numpy()
+
+
+def weighted_orbital_norms(orbitals, grid_geometry, *, use_jax_block_kernels=False):
+    S = weighted_overlap_matrix(orbitals, grid_geometry)
+    return np.sqrt(np.clip(np.diag(S), 0.0, None))
+
+
+def weighted_orthonormalize_orbitals(orbitals, grid_geometry, *, require_full_rank=True, rank_tolerance=1.0e-12,
+                                     use_jax_block_kernels=False):
+    """ks/eigensolver.py:292-341."""
+    dg = device_grid(grid_geometry)
+    blk = np.asarray(orbitals, dtype=np.float64)
+    if blk.shape[0] == 0:
+        return blk.copy()
+    x = torch.from_numpy(np.ascontiguousarray(blk.reshape(blk.shape[0], -1))).to(dg.device)
+    out = orthonormalize_device(x, dg.weights(), rank_tolerance, require_full_rank)
+    return out.cpu().numpy().reshape((-1,) + dg.shape)
