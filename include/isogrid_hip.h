@@ -176,6 +176,17 @@ int ig_nonlocal_apply_f32(float* out, long long ld, int nblk, int nx, int ny, in
                           const int* box, const double* P, int nproj, const double* d,
                           hipStream_t stream);
 
+/* ------------------------------------------------------------------ orbital density */
+
+/* rho (=|+=) sum_m occ[m] psi_m^2 over an orbital-major block (ld = orbital stride),
+ * FP64 accumulation; occ is a device array of nblk doubles.
+ * Replaces: scf/driver.py:663-676 (_build_density_from_occupied_orbitals),
+ * ks/static_hamiltonian.py:199-235 (build_orbital_density / build_total_density). */
+int ig_density_f64(const double* psi, long long ld, int nblk, long long N, const double* occ, double* rho,
+                   int accumulate, hipStream_t stream);
+int ig_density_f32(const float* psi, long long ld, int nblk, long long N, const double* occ, double* rho,
+                   int accumulate, hipStream_t stream);
+
 /* ------------------------------------------------------------------ multipoles */
 
 /* Raw Cartesian moments sum_p rho w [frac] x^a y^b z^c of degree <= order (<= 4) about

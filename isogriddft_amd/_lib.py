@@ -48,6 +48,8 @@ SIGNATURES = {
     "ig_nonlocal_project_f32": [P, L, I, I, I, I, P, P, I, P, P, P, P, P, P],
     "ig_nonlocal_apply_f64": [P, L, I, I, I, I, P, P, I, P, P],
     "ig_nonlocal_apply_f32": [P, L, I, I, I, I, P, P, I, P, P],
+    "ig_density_f64": [P, L, I, L, P, P, I, P],
+    "ig_density_f32": [P, L, I, L, P, P, I, P],
     "ig_multipole_moments_f64": [P, P, P, P, L, P, I, P, P, P],
     "ig_boundary_values_f64": [P, I, I, I, P, I, P, P, P],
     "ig_cell_fit_f64": [P, P, I, I, I, P, I, P, P],

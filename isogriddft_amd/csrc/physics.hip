@@ -317,6 +317,23 @@ __global__ __launch_bounds__(IG_THREADS) void proj_apply_kernel(T* __restrict__ 
   }
 }
 
+// ------------------------------------------------------------------ orbital density
+
+// rho(t) (=|+=) sum_m occ[m] psi_m(t)^2, FP64 accumulation and output.
+template <typename T>
+__global__ __launch_bounds__(IG_THREADS) void density_kernel(const T* __restrict__ psi, long long ld, int nblk,
+                                                             long long N, const double* __restrict__ occ,
+                                                             double* __restrict__ rho, int accumulate) {
+  for (long long t = (long long)blockIdx.x * IG_THREADS + threadIdx.x; t < N; t += (long long)gridDim.x * IG_THREADS) {
+    double s = accumulate ? rho[t] : 0.0;
+    for (int m = 0; m < nblk; ++m) {
+      const double v = (double)psi[(long long)m * ld + t];
+      s = fma(occ[m] * v, v, s);
+    }
+    rho[t] = s;
+  }
+}
+
 // ------------------------------------------------------------------ multipoles
 
 template <int ORDER>
@@ -648,6 +665,19 @@ int ig_nonlocal_work_size(int nblk, int nproj) { return 64 * nblk * nproj + 8; }
 
 IG_PROJECT(double, f64)
 IG_PROJECT(float, f32)
+
+#define IG_DENSITY(SFX, T)                                                                                 \
+  int ig_density_##SFX(const T* psi, long long ld, int nblk, long long N, const double* occ, double* rho,   \
+                       int accumulate, hipStream_t s) {                                                    \
+    if (!psi || !occ || !rho || nblk <= 0 || N <= 0 || ld < N) return IG_EINVAL;                           \
+    hipLaunchKernelGGL(density_kernel<T>, dim3(grid1d(N)), dim3(IG_THREADS), 0, s, psi, ld, nblk, N, occ,  \
+                       rho, accumulate);                                                                   \
+    IG_CHECK_LAUNCH();                                                                                     \
+    return IG_OK;                                                                                          \
+  }
+IG_DENSITY(f64, double)
+IG_DENSITY(f32, float)
+#undef IG_DENSITY
 
 int ig_multipole_moments_f64(const double* rho, const double* w, const double* frac, const double* xyz, long long N,
                              const double* center, int order, double* work, double* out, hipStream_t s) {

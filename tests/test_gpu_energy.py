@@ -1,0 +1,77 @@
+"""GPU energy components and densities against the FP64 oracle.
+
+Bar: every component within 1e-8 Ha (north-star energy bar); densities 1e-14
+relative.  The Poisson solve runs to rtol 1e-13 so E_H carries no Krylov error.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from isogriddft_amd import configs, energy
+from isogriddft_amd.device import DeviceGrid
+from oracle import energy as oenergy
+from oracle import grid as ogrid
+from oracle import kinetic as okin
+from oracle import poisson as opoisson
+from oracle import pseudo as opseudo
+from oracle import xc as oxc
+
+pytestmark = pytest.mark.gpu
+
+
+def _orbitals(geom, k, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(k):
+        c = rng.uniform(-1.0, 1.0, 3)
+        a = rng.uniform(0.4, 1.2)
+        f = np.exp(-a * ((geom.x_points - c[0]) ** 2 + (geom.y_points - c[1]) ** 2 + (geom.z_points - c[2]) ** 2))
+        out.append(f / np.sqrt(np.sum(f * f * geom.cell_volumes)))
+    return np.array(out)
+
+
+def _cases():
+    yield "legacy25x27x29/H2O", ogrid.build_legacy_geometry(shape=(25, 27, 29), half_extents=(6.0, 6.0, 7.0),
+                                                           stretch=(2.5, 2.5, 2.8)), ogrid.h2o_case()
+    yield "synthetic22x24x26/H2", configs.build_synthetic_geometry(shape=(22, 24, 26)), ogrid.H2_CASE
+
+
+def test_density_kernel():
+    g = configs.build_synthetic_geometry(shape=(10, 11, 12))
+    dg = DeviceGrid(g)
+    psi = _orbitals(g, 3, 1)
+    occ = (1.0, 0.5, 2.0)
+    for dt in (torch.float64, torch.float32):
+        rho = energy.density_device(dg, torch.from_numpy(psi).to("cuda", dt), occ).cpu().numpy()
+        ref = oenergy.density(psi.astype(np.float32 if dt == torch.float32 else np.float64).astype(np.float64), occ)
+        assert np.max(np.abs(rho - ref.reshape(-1))) <= 1e-14 * np.max(ref)
+
+
+def test_single_point_components_match_oracle():
+    for name, geom, case in _cases():
+        up = _orbitals(geom, 2, 3)
+        dn = _orbitals(geom, 1, 4)
+        occ = energy.SpinOccupations(2.0, 1.0, (1.0, 1.0), (1.0,))
+        dg = DeviceGrid(geom)
+        t = lambda a: torch.from_numpy(a).cuda()  # noqa: E731
+        got = energy.energy_components_device(dg, t(up), t(dn), occ, case, poisson_tolerance=1e-13)
+        w = geom.cell_volumes
+        ru = oenergy.renormalize(oenergy.density(up, occ.occupations_up), 2.0, w)
+        rd = oenergy.renormalize(oenergy.density(dn, occ.occupations_down), 1.0, w)
+        rho = ru + rd
+        kin = sum(oenergy.expectation(p, okin.kinetic_operator(p, geom), w) for p in np.concatenate([up, dn]))
+        eloc = float(np.sum(rho * opseudo.local_ionic_potential(case, geom) * w))
+        enl = sum(oenergy.expectation(p, opseudo.nonlocal_action(p, case, geom), w) for p in np.concatenate([up, dn]))
+        pot, _, _, _ = opoisson.solve_poisson(geom, rho, order=2, method="direct")
+        eh = opoisson.hartree_energy(rho, geom, pot)
+        eps, _, _ = oxc.lsda_pw92(ru, rd)
+        exc = float(np.sum(rho * eps * w))
+        eii = oenergy.ion_ion(case)
+        ref = dict(kinetic=kin, local_ionic=eloc, nonlocal_ionic=enl, hartree=eh, xc=exc, ion_ion_repulsion=eii)
+        for key, val in ref.items():
+            print(name, key, getattr(got, key), val, getattr(got, key) - val)
+            assert abs(getattr(got, key) - val) <= 1e-8, (name, key)
+        assert abs(got.total - sum(ref.values())) <= 1e-8
+        if "H2O" in name:
+            assert abs(got.nonlocal_ionic) > 1e-6  # the O s-projector contributes
