@@ -47,6 +47,9 @@ def parse():
                     help="c2: 96^3, k=16 (default); c5: 224^3, k=48 (benzene-size block)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-solvers", action="store_true",
+                    help="skip the eigensolve + Poisson timing (FP64 vs FP32 + FP64 refinement)")
+    ap.add_argument("--eig-tol", type=float, default=1e-3)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (the scale for PMC byte counters)")
     return ap.parse_args()
@@ -81,6 +84,61 @@ def cpu_baseline(geom, psi_np, veff_np, seconds):
         "sample": f"FP64 NumPy oracle (ops/kinetic.py restatement), {done} orbital applies "
                   f"({blocks:.2f} blocks of {k}) on the 96^3 grid in {dt:.1f} s, single thread",
     }
+
+
+def solver_leg(geom, dg, cfg, veff64, eig_tol):
+    """Poisson (order-2 multipoles, rtol 1e-8) on the synthetic charges, then the lowest-k
+    eigenproblem of T + V + v_H (production closure, working subspace 12k + 8, residual
+    tolerance eig_tol), each in FP64 and in FP32 with FP64 refinement."""
+    import torch
+
+    from isogriddft_amd import configs, eigensolver, poisson
+    from isogriddft_amd.hamiltonian import BlockHamiltonian
+
+    rho = configs.build_synthetic_density(geom, cfg, device="cuda").to(torch.float64)
+    out = {}
+    pots = {}
+    for prec in ("fp64", "mixed"):
+        poisson.solve_poisson_device(dg, rho, tolerance=1e-8, precision=prec)  # warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sol = poisson.solve_poisson_device(dg, rho, tolerance=1e-8, precision=prec)
+        torch.cuda.synchronize()
+        out[f"poisson_{prec}_s"] = round(time.perf_counter() - t0, 4)
+        out[f"poisson_{prec}_iterations"] = sol["iterations"]
+        pots[prec] = sol["potential"]
+    out["poisson_mixed_vs_fp64_max_rel"] = float((pots["mixed"] - pots["fp64"]).abs().max()
+                                                 / pots["fp64"].abs().max())
+    # a neutral molecule-like potential: two Z = 3 soft-Coulomb centres plus the Hartree
+    # potential of the 6-electron synthetic density (trial_fix: exactly W-self-adjoint)
+    xyz = dg.xyz()
+    vion = sum(-3.0 / torch.sqrt(0.3 + (xyz[0] - c[0]) ** 2 + (xyz[1] - c[1]) ** 2 + (xyz[2] - c[2]) ** 2)
+               for c in cfg.centers)
+    H = BlockHamiltonian(dg, vion + pots["fp64"].reshape(-1), closure="trial_fix")
+    k = cfg.k
+    m = eigensolver.reference_block_size(k)
+    atoms = [("H", c) for c in cfg.centers]
+    X0 = eigensolver.default_guess_block(geom, atoms, m, dg)
+    ev = {}
+    for prec in ("fp64", "mixed"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = eigensolver.block_subspace_iteration(H, dg.weights(), X0, k, eig_tol, 400, precision=prec,
+                                                   profile=True)
+        torch.cuda.synchronize()
+        out[f"eig_{prec}_s"] = round(time.perf_counter() - t0, 4)
+        out[f"eig_{prec}_iterations"] = res.iterations
+        out[f"eig_{prec}_converged"] = bool(res.converged)
+        ev[prec] = np.asarray(res.eigenvalues)
+        out[f"eig_{prec}_phase_s"] = {key: round(v, 4) for key, v in res.timings.items()}
+    out["eig_fp32_iterations"] = res.fp32_iterations
+    out["eig_mixed_vs_fp64_max_abs_Ha"] = float(np.max(np.abs(ev["mixed"] - ev["fp64"])))
+    out["eigensolve_plus_poisson_fp64_s"] = round(out["eig_fp64_s"] + out["poisson_fp64_s"], 4)
+    out["eigensolve_plus_poisson_mixed_s"] = round(out["eig_mixed_s"] + out["poisson_mixed_s"], 4)
+    out["workload"] = (f"{dg.shape[0]}^3 grid, Poisson: {cfg.n_charges} Gaussian charges, order-2 multipoles, "
+                       f"rtol 1e-8; eigensolve: k={k}, working subspace {m}, residual tol {eig_tol}, "
+                       "trial_fix closure, V = two Z=3 soft-Coulomb centres + v_H")
+    return out
 
 
 def main():
@@ -183,6 +241,8 @@ def main():
             "wall_s_timed_region": round(wall, 4),
         },
     }
+    if not args.no_solvers:
+        line["eigensolve_poisson"] = solver_leg(geom, dg, cfg, veff64, args.eig_tol)
     if not args.no_cpu:
         psi_np = psi0[: k].cpu().numpy()
         line["cpu_baseline"] = cpu_baseline(geom, psi_np, veff64.cpu().numpy().reshape(dg.shape), args.cpu_seconds)
