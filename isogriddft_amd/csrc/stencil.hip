@@ -454,6 +454,422 @@ int launch_monitor_t(const MonParams<T>& base, hipStream_t stream) {
   return IG_OK;
 }
 
+// ---------------------------------------------------------------------------------
+// z-vectorised forward trial_fix stencil.  Each lane owns V = 16 / sizeof(T)
+// consecutive z points (one 128-bit access), a tile row of TZ = 16 V points is 16
+// lanes (one DPP row), the tile is 16 rows.  Workgroup = 5 waves: waves 0-3 compute
+// the 16 x TZ interior points; wave 4 computes the halo fluxes the interior needs (F_y
+// on rows -1 and 16 with lanes 0-31, F_z at z = k0-1 and k0+TZ with lanes 32-63).  Per
+// x step there is one barrier: the input ring has 4 planes (s-1 .. s+2 in use, plane
+// s+3 written after the barrier into the slot of s-1), the staged F_y / F_z halo are
+// double buffered, z neighbours of F_z come from the adjacent lane by DPP.
+// Output plane o = s - 1 is written at step s.
+template <typename T>
+struct ZVec;
+template <>
+struct ZVec<float> {
+  typedef float t __attribute__((ext_vector_type(4)));
+  static constexpr int V = 4;
+};
+template <>
+struct ZVec<double> {
+  typedef double t __attribute__((ext_vector_type(2)));
+  static constexpr int V = 2;
+};
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+__device__ __forceinline__ typename ZVec<T>::t zld(rsrc_t r, unsigned off) {
+  return __builtin_bit_cast(typename ZVec<T>::t, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
+}
+template <typename T>
+__device__ __forceinline__ void zst(rsrc_t r, unsigned off, typename ZVec<T>::t v) {
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, (int)off, 0, 0);
+}
+// Value of the lane one to the left (CTRL = 0x111, row_shr:1) or right (0x101,
+// row_shl:1) inside its 16-lane row; lanes without a source get 0 (replaced by the
+// staged halo value by the caller).
+template <int CTRL>
+__device__ __forceinline__ float zdpp(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double zdpp(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffull), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo);
+}
+
+constexpr int ZV_NS = 4;
+
+// Tile shape per precision: FP32 rows of 8 lanes x 4 points (TZ = 32), 24 rows, 4 waves
+// (3 interior + 1 halo); FP64 rows of 16 lanes x 2 points (TZ = 32), 16 rows, 5 waves.
+template <typename T>
+struct ZShape;
+template <>
+struct ZShape<float> {
+  static constexpr int LZ = 8, TY = 24, NT = 256;
+};
+template <>
+struct ZShape<double> {
+  static constexpr int LZ = 16, TY = 16, NT = 320;
+};
+
+template <typename T, int NB, int WPE>
+__global__ __launch_bounds__(ZShape<T>::NT) __attribute__((amdgpu_waves_per_eu(WPE)))
+void monitor_zv_kernel(MonParams<T> p) {
+  using VT = typename ZVec<T>::t;
+  constexpr int V = ZVec<T>::V;
+  constexpr int LZ = ZShape<T>::LZ, TY = ZShape<T>::TY, NT = ZShape<T>::NT, TZ = LZ * V;
+  constexpr int PY = TY + 4;          // ring rows: tile rows with a 2-row halo
+  // A ring plane keeps the TZ-point rows packed back to back (a 16-lane LDS pass reads two
+  // consecutive 128-byte rows: conflict free) and the 2-point z halo of every row in a
+  // separate array of 4 values per row (k0-2, k0-1, k0+TZ, k0+TZ+1).
+  constexpr int PLANE = PY * TZ + V;  // + a spare vector absorbing stores of threads without a ring element
+  constexpr int HPL = PY * 4 + 4;     // halo values per plane + a spare
+  constexpr int FYR = TY + 2;         // F_y rows -1 .. TY
+  constexpr unsigned S = sizeof(T);
+  static_assert(TY * LZ + 2 * LZ + 2 * TY == NT, "interior, y-halo and z-halo units fill the workgroup");
+  static_assert(PY * LZ <= NT && PY * 4 <= NT, "one ring vector and one ring halo scalar per thread");
+  static_assert(16 % LZ == 0, "tile rows must tile the 16-lane DPP rows");
+
+  __shared__ __attribute__((aligned(16))) T s_ring[NB][ZV_NS * PLANE];
+  __shared__ T s_rh[NB][ZV_NS * HPL];
+  __shared__ __attribute__((aligned(16))) T s_fy[2][NB][FYR * TZ];
+  __shared__ T s_fzh[2][NB][2 * TY];
+
+  const int ngroups = (p.nblk + NB - 1) / NB;
+  const int b = blockIdx.x;
+  const int xcd = b & 7;
+  const int r = b >> 3;
+  const int grp = r % ngroups;
+  const int upx = (p.units + 7) / 8;
+  const int u = xcd * upx + r / ngroups;
+  if (u >= p.units) return;
+  const int tzi = u % p.ntz;
+  const int tyi = (u / p.ntz) % p.nty;
+  const int xci = u / (p.ntz * p.nty);
+
+  const int nx = p.nx, ny = p.ny, nz = p.nz;
+  const unsigned plane_bytes = (unsigned)(ny * nz) * S;
+  const int j0 = min(tyi * TY, ny - TY);
+  const int k0 = min(tzi * TZ, nz - TZ);
+  const int x0 = xci * p.xc;
+  const int x1 = min(x0 + p.xc, nx);
+  const long long N = (long long)nx * ny * nz;
+
+  rsrc_t rin[NB], rout[NB];
+  bool orb_ok[NB];
+#pragma unroll
+  for (int n = 0; n < NB; ++n) {
+    const int orb = grp * NB + n;
+    orb_ok[n] = orb < p.nblk;
+    const int oc = orb_ok[n] ? orb : p.nblk - 1;
+    rin[n] = mkbuf(p.in + (long long)oc * p.ld, N);
+    rout[n] = mkbuf(p.out + (long long)oc * p.ld, N);
+  }
+  const rsrc_t rxx = mkbuf(p.coef, N), ryy = mkbuf(p.coef + N, N), rzz = mkbuf(p.coef + 2 * N, N);
+  const rsrc_t rxy = mkbuf(p.coef + 3 * N, N), rxz = mkbuf(p.coef + 4 * N, N), ryz = mkbuf(p.coef + 5 * N, N);
+  const rsrc_t rij = mkbuf(p.invj, N);
+  const rsrc_t rv = mkbuf(p.veff ? p.veff : p.invj, p.veff ? N : 0);
+
+  const int tid = threadIdx.x;
+  // ---- ring loading role (every thread): at most one vector and one halo scalar per plane
+  unsigned ld_off = BAD;
+  int ld_lds = PY * TZ;
+  if (tid < PY * LZ) {
+    const int lr = tid / LZ, llz = tid % LZ;
+    const int lj = j0 - 2 + lr;
+    ld_lds = lr * TZ + V * llz;
+    if (lj >= 0 && lj < ny) ld_off = (unsigned)(lj * nz + k0 + V * llz) * S;
+  }
+  unsigned hs_off = BAD;
+  int hs_lds = PY * 4;
+  if (tid < PY * 4) {
+    const int hr = tid >> 2, h = tid & 3;
+    const int k = h < 2 ? k0 - 2 + h : k0 + TZ + (h - 2);
+    const int j = j0 - 2 + hr;
+    hs_lds = tid;
+    if (j >= 0 && j < ny && k >= 0 && k < nz) hs_off = (unsigned)(j * nz + k) * S;
+  }
+
+  // ---- compute role
+  const bool is_int = tid < TY * LZ;
+  const bool is_yh = !is_int && tid < FYR * LZ;
+  const bool is_zh = tid >= FYR * LZ;
+  int row, lz, side = 0;
+  if (is_int) {
+    row = tid / LZ;
+    lz = tid % LZ;
+  } else if (is_yh) {
+    const int q = tid - TY * LZ;
+    row = (q / LZ) ? TY : -1;
+    lz = q % LZ;
+  } else {
+    const int q = tid - FYR * LZ;
+    row = q >> 1;
+    side = q & 1;
+    lz = 0;
+  }
+  const int gj = j0 + row;
+  const bool jok = gj >= 0 && gj < ny;
+  const unsigned voff = (!is_zh && jok) ? (unsigned)(gj * nz + k0 + V * lz) * S : BAD;
+  const unsigned ooff = is_int ? voff : BAD;
+  const int cidx = (row + 2) * TZ + V * lz;
+  const int hrow = (row + 2) * 4;  // halo values of the unit's ring row
+  const int zl_idx = lz == 0 ? hrow + 1 : 0;
+  const int zr_idx = lz == LZ - 1 ? hrow + 2 : 0;
+  const int kz = side ? k0 + TZ : k0 - 1;
+  const unsigned zoff = (is_zh && jok && kz >= 0 && kz < nz) ? (unsigned)(gj * nz + kz) * S : BAD;
+  // z-halo point: its value and y neighbours live in the halo array, its inner z
+  // neighbour in the packed row
+  const int zh_c = hrow + 1 + side;
+  const int zh_out = hrow + (side ? 3 : 0);
+  const int zh_in = (row + 2) * TZ + (side ? TZ - 1 : 0);
+  unsigned bmask = 0;
+#pragma unroll
+  for (int e = 0; e < V; ++e) {
+    const int k = k0 + V * lz + e;
+    if (gj == 0 || gj == ny - 1 || k == 0 || k == nz - 1) bmask |= 1u << e;
+  }
+
+  auto pbytes = [&](int q) { return (q >= 0 && q < nx) ? (unsigned)q * plane_bytes : BAD; };
+
+  VT stg[NB];
+  T stgh[NB];
+  auto fetch_plane = [&](int q) {
+    const unsigned pb = pbytes(q);
+#pragma unroll
+    for (int n = 0; n < NB; ++n) {
+      stg[n] = zld<T>(rin[n], pb + ld_off);
+      stgh[n] = bld(rin[n], pb + hs_off, T(0));
+    }
+  };
+  auto store_plane = [&](int slot) {
+#pragma unroll
+    for (int n = 0; n < NB; ++n) {
+      *reinterpret_cast<VT*>(&s_ring[n][slot * PLANE + ld_lds]) = stg[n];
+      s_rh[n][slot * HPL + hs_lds] = stgh[n];
+    }
+  };
+  // flux coefficients of the unit, two planes in flight (set = parity of plane - sb)
+  VT gxx[2], gyy[2], gzz[2], gxy[2], gxz[2], gyz[2];
+  T zxz[2], zyz[2], zzz[2];
+  VT oij, ov;  // 1/J and V at the next output plane
+  auto fetch_geom = [&](int q, int w) {
+    const unsigned pb = pbytes(q);
+    gxx[w] = zld<T>(rxx, pb + voff);
+    gyy[w] = zld<T>(ryy, pb + voff);
+    gzz[w] = zld<T>(rzz, pb + voff);
+    gxy[w] = zld<T>(rxy, pb + voff);
+    gxz[w] = zld<T>(rxz, pb + voff);
+    gyz[w] = zld<T>(ryz, pb + voff);
+    zxz[w] = bld(rxz, pb + zoff, T(0));
+    zyz[w] = bld(ryz, pb + zoff, T(0));
+    zzz[w] = bld(rzz, pb + zoff, T(0));
+  };
+  auto fetch_out = [&](int o) {
+    const unsigned pb = pbytes(o);
+    oij = zld<T>(rij, pb + ooff);
+    ov = zld<T>(rv, pb + ooff);
+  };
+
+  VT fx[NB][ZV_NS];   // F_x of the last planes, indexed by (plane - sb) mod 4
+  VT fz[NB];          // F_z of plane s (interior)
+  VT pc[NB][2];       // psi at the owned points of planes s and s - 1 (parity of s - sb)
+  VT dyz[NB][2];      // y/z divergence of planes s and s - 1
+#pragma unroll
+  for (int n = 0; n < NB; ++n) {
+#pragma unroll
+    for (int a = 0; a < ZV_NS; ++a) fx[n][a] = VT(0);
+    fz[n] = pc[n][0] = pc[n][1] = dyz[n][0] = dyz[n][1] = VT(0);
+  }
+
+  const int sb = x0 - 1;  // steps sb .. x1; output o = s - 1 in [x0, x1)
+  // ring slot of plane q: (q - sb + 1) mod 4
+  for (int q = sb - 1; q <= sb + 2; ++q) {
+    fetch_plane(q);
+    store_plane(q - sb + 1);
+  }
+  fetch_plane(sb + 3);
+  fetch_geom(sb, 0);
+  fetch_geom(sb + 1, 1);
+  fetch_out(sb - 1);
+  const T sc = p.scale;
+  __syncthreads();
+
+  for (int s0 = sb; s0 <= x1; s0 += ZV_NS) {
+#pragma unroll
+    for (int U = 0; U < ZV_NS; ++U) {
+      const int s = s0 + U;
+      if (s > x1) break;
+      const int sm = U % ZV_NS, s0l = (U + 1) % ZV_NS, sp = (U + 2) % ZV_NS;  // slots of s-1, s, s+1
+      const int fb = U & 1;                                                   // staging buffer / parity
+      // ---------------- phase B: fluxes of plane s
+      auto vgrad = [&](int n, VT& c, VT& gx, VT& gy, VT& gz) {
+        const T* pl = &s_ring[n][s0l * PLANE];
+        c = *reinterpret_cast<const VT*>(&pl[cidx]);
+        const VT ym = *reinterpret_cast<const VT*>(&pl[cidx - TZ]);
+        const VT yp = *reinterpret_cast<const VT*>(&pl[cidx + TZ]);
+        const VT xm = *reinterpret_cast<const VT*>(&s_ring[n][sm * PLANE + cidx]);
+        const VT xp = *reinterpret_cast<const VT*>(&s_ring[n][sp * PLANE + cidx]);
+        // z neighbours: from the adjacent lane (DPP); the tile-row edge lanes read the
+        // ring halo (all other lanes read one broadcast address: no bank conflicts)
+        const T hl = s_rh[n][s0l * HPL + zl_idx], hr = s_rh[n][s0l * HPL + zr_idx];
+        const T nl = zdpp<0x111>(c[V - 1]), nr = zdpp<0x101>(c[0]);
+        const T zl = lz == 0 ? hl : nl, zr = lz == LZ - 1 ? hr : nr;
+        gx = xp - xm;
+        gy = yp - ym;
+#pragma unroll
+        for (int e = 0; e < V; ++e) gz[e] = (e + 1 < V ? c[e + 1] : zr) - (e > 0 ? c[e - 1] : zl);
+      };
+      if (is_int) {
+#pragma unroll
+        for (int n = 0; n < NB; ++n) {
+          VT c, gx, gy, gz, f1, f2, f3;
+          vgrad(n, c, gx, gy, gz);
+#pragma unroll
+          for (int e = 0; e < V; ++e) {
+            f1[e] = flux3(gxx[fb][e], gx[e], gxy[fb][e], gy[e], gxz[fb][e], gz[e]);
+            f2[e] = flux3(gxy[fb][e], gx[e], gyy[fb][e], gy[e], gyz[fb][e], gz[e]);
+            f3[e] = flux3(gxz[fb][e], gx[e], gyz[fb][e], gy[e], gzz[fb][e], gz[e]);
+          }
+          *reinterpret_cast<VT*>(&s_fy[fb][n][(row + 1) * TZ + V * lz]) = f2;
+          fx[n][(U + 1) % ZV_NS] = f1;
+          fz[n] = f3;
+          pc[n][fb] = c;
+        }
+      } else if (is_yh) {
+#pragma unroll
+        for (int n = 0; n < NB; ++n) {
+          VT c, gx, gy, gz, f2;
+          vgrad(n, c, gx, gy, gz);
+#pragma unroll
+          for (int e = 0; e < V; ++e) f2[e] = flux3(gxy[fb][e], gx[e], gyy[fb][e], gy[e], gyz[fb][e], gz[e]);
+          *reinterpret_cast<VT*>(&s_fy[fb][n][(row + 1) * TZ + V * lz]) = f2;
+        }
+      } else {
+#pragma unroll
+        for (int n = 0; n < NB; ++n) {
+          const T* hp = &s_rh[n][s0l * HPL];
+          const T gx = s_rh[n][sp * HPL + zh_c] - s_rh[n][sm * HPL + zh_c];
+          const T gy = hp[zh_c + 4] - hp[zh_c - 4];
+          const T inner = s_ring[n][s0l * PLANE + zh_in];
+          const T gz = side ? hp[zh_out] - inner : inner - hp[zh_out];
+          s_fzh[fb][n][2 * row + side] = flux3(zxz[fb], gx, zyz[fb], gy, zzz[fb], gz);
+        }
+      }
+      // flux coefficients of plane s + 2 into the set just used
+      fetch_geom(s + 2, fb);
+      __syncthreads();
+      // ---------------- phase C: divergence of plane s, output plane o = s - 1
+      const int o = s - 1;
+      const unsigned ob = (o >= x0 && o < x1) ? (unsigned)o * plane_bytes : BAD;
+      const bool obnd = (o == 0 || o == nx - 1);
+      if (is_int) {
+#pragma unroll
+        for (int n = 0; n < NB; ++n) {
+          const VT fyp = *reinterpret_cast<const VT*>(&s_fy[fb][n][(row + 2) * TZ + V * lz]);
+          const VT fym = *reinterpret_cast<const VT*>(&s_fy[fb][n][row * TZ + V * lz]);
+          const T hl = s_fzh[fb][n][2 * row], hr = s_fzh[fb][n][2 * row + 1];
+          const T nl = zdpp<0x111>(fz[n][V - 1]);
+          const T nr = zdpp<0x101>(fz[n][0]);
+          const T fzl = lz == 0 ? hl : nl;
+          const T fzr = lz == LZ - 1 ? hr : nr;
+          VT d;
+#pragma unroll
+          for (int e = 0; e < V; ++e)
+            d[e] = (fyp[e] - fym[e]) + ((e + 1 < V ? fz[n][e + 1] : fzr) - (e > 0 ? fz[n][e - 1] : fzl));
+          dyz[n][fb] = d;
+          const VT dx = fx[n][(U + 1) % ZV_NS] - fx[n][(U + 3) % ZV_NS];
+          const VT dyzo = dyz[n][fb ^ 1];
+          const VT po = pc[n][fb ^ 1];
+          VT v;
+#pragma unroll
+          for (int e = 0; e < V; ++e) {
+            const T lap = dx[e] + dyzo[e];
+            T val = fma(ov[e], po[e], sc * (lap * oij[e]));
+            if (p.interior_only && (obnd || ((bmask >> e) & 1u))) val = T(0);
+            v[e] = val;
+          }
+          zst<T>(rout[n], (orb_ok[n] ? ob : BAD) + ooff, v);
+        }
+      }
+      // commit plane s + 3 into the slot of plane s - 1, then request the next data
+      store_plane(sm);
+      fetch_plane(s + 4);
+      fetch_out(s);
+    }
+  }
+}
+
+template <typename T, int NB, int WPE = 1>
+int launch_monitor_zv_t(const MonParams<T>& base, hipStream_t stream) {
+  constexpr int V = ZVec<T>::V;
+  constexpr int TZ = ZShape<T>::LZ * V, TY = ZShape<T>::TY, NT = ZShape<T>::NT;
+  MonParams<T> p = base;
+  if ((long long)p.nx * p.ny * p.nz * (long long)sizeof(T) >= (1LL << 30)) return IG_EUNSUPPORTED;
+  if (p.nz % V != 0 || p.nz < TZ || p.ny < TY || p.nx < 1 || p.ld % V != 0) return IG_EUNSUPPORTED;
+  // 128-bit accesses: every operand base must be 16-byte aligned
+  auto al = [](const void* q) { return q == nullptr || (reinterpret_cast<uintptr_t>(q) & 15u) == 0; };
+  if (!al(p.in) || !al(p.out) || !al(p.coef) || !al(p.invj) || !al(p.veff)) return IG_EUNSUPPORTED;
+  p.nty = (p.ny + TY - 1) / TY;
+  p.ntz = (p.nz + TZ - 1) / TZ;
+  const long long groups = (p.nblk + NB - 1) / NB;
+  if (p.xc <= 0) {
+    static int occ = -1;
+    if (occ < 0) {
+      int n = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, monitor_zv_kernel<T, NB, WPE>, NT, 0) != hipSuccess || n < 1)
+        n = 1;
+      occ = n;
+    }
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+      cus = 256;
+    const long long capacity = (long long)occ * cus;
+    const long long tiles = (long long)p.nty * p.ntz;
+    const int maxc = p.nx / 8 > 1 ? p.nx / 8 : 1;
+    long long best = -1;
+    int best_c = 1;
+    for (int c = 1; c <= maxc; ++c) {
+      const int xc = (p.nx + c - 1) / c;
+      const long long blocks = 8LL * ((tiles * c + 7) / 8) * groups;
+      const long long cost = ((blocks + capacity - 1) / capacity) * (xc + 6);
+      if (best < 0 || cost < best) {
+        best = cost;
+        best_c = c;
+      }
+    }
+    p.xc = (p.nx + best_c - 1) / best_c;
+  }
+  if (p.xc > p.nx) p.xc = p.nx;
+  p.nxc = (p.nx + p.xc - 1) / p.xc;
+  p.units = p.nty * p.ntz * p.nxc;
+  const long long blocks = 8LL * groups * ((p.units + 7) / 8);
+  if (blocks <= 0 || blocks > 0x7fffffffLL) return IG_EINVAL;
+  hipLaunchKernelGGL((monitor_zv_kernel<T, NB, WPE>), dim3((unsigned)blocks), dim3(NT), 0, stream, p);
+  IG_CHECK_LAUNCH();
+  return IG_OK;
+}
+
+template <typename T>
+int launch_monitor_zv(const MonParams<T>& p, int nb, hipStream_t s) {
+  if (nb <= 0 || (nb % 10) > p.nblk) nb = p.nblk >= 2 ? 2 : 1;
+  // nb = NB + 10 * WPE (waves per SIMD the register allocation must allow; 0 = no bound)
+  switch (nb) {
+    case 1: return launch_monitor_zv_t<T, 1>(p, s);
+    case 2: return launch_monitor_zv_t<T, 2>(p, s);
+    case 31: return launch_monitor_zv_t<T, 1, 3>(p, s);
+    case 41: return launch_monitor_zv_t<T, 1, 4>(p, s);
+    case 32: return launch_monitor_zv_t<T, 2, 3>(p, s);
+    default: return IG_EUNSUPPORTED;
+  }
+}
+
+
 #define IG_MON_CASE(TY_, TZ_, NB_, PD_)                                                            \
   if (ty == TY_ && tz == TZ_ && nb == NB_ && pd == PD_)                                            \
     return closure ? launch_monitor_t<T, TY_, TZ_, 1, false, NB_, PD_>(p, s)                       \
@@ -466,6 +882,11 @@ template <typename T>
 int launch_monitor(const MonParams<T>& p, int closure, int transpose, int tile, hipStream_t s) {
   if (transpose) return closure ? launch_monitor_t<T, 8, 32, 1, true>(p, s) : launch_monitor_t<T, 8, 32, 0, true>(p, s);
   if (tile < 0) return IG_EINVAL;
+  if (tile >= 7000000) return closure ? IG_EUNSUPPORTED : launch_monitor_zv<T>(p, tile - 7000000, s);
+  if (tile == 0 && closure == 0) {
+    const int st = launch_monitor_zv<T>(p, 0, s);
+    if (st != IG_EUNSUPPORTED) return st;
+  }
   int pd = tile / 100000;
   int nb = (tile / 10000) % 10;
   const int t2 = tile % 10000;
