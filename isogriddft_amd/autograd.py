@@ -1,0 +1,167 @@
+"""Adjoints of the inner-loop operators behind torch.autograd.Function.
+
+The reference has no derivatives; these are the derivative paths a gradient-based
+caller (geometry optimisation, potential inversion, learned functionals) needs, built
+on the same HIP kernels as the forward path:
+
+* ``HPsi``            H psi = -1/2 Lap psi + V psi for a block.  Backward uses the exact
+                      Euclidean transpose of the stencil (the TR instantiation of the
+                      monitor kernel, the transposed legacy kernel): d/dpsi = H^T g,
+                      d/dV = sum_m psi_m g_m.
+* ``RitzValues``      lowest-k eigenvalues of the fixed-potential operator T + V (block
+                      iteration); backward is first-order perturbation theory
+                      (Hellmann-Feynman): d eps_i / d V_p = w_p psi_i(p)^2 with
+                      W-normalised Ritz vectors, i.e. dE/dv_ext for E = sum_i g_i eps_i.
+* ``IonicEnergy``     E_ion(R) = E_loc + E_nl + E_II at frozen density and orbitals as a
+                      function of the atomic positions; backward gives dE/dR (minus the
+                      Hellmann-Feynman forces): ig_vloc_gth_grad_f64 for the local part,
+                      projector-position derivative fields for the nonlocal part, the
+                      Coulomb sum for E_II.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import pseudo
+from .device import DeviceGrid
+from .kinetic import apply_operator
+from .reductions import wdot
+
+
+class HPsi(torch.autograd.Function):
+    """out = -1/2 Lap(psi) + veff * psi on one grid (psi: (k, *shape) or (*shape))."""
+
+    @staticmethod
+    def forward(ctx, psi, veff, dg: DeviceGrid, closure="production"):
+        ctx.dg = dg
+        ctx.closure = closure
+        ctx.save_for_backward(psi, veff)
+        v = veff.reshape(dg.shape)
+        return apply_operator(dg, psi.detach(), scale=-0.5, closure=closure, veff=v.detach())
+
+    @staticmethod
+    def backward(ctx, g):
+        psi, veff = ctx.saved_tensors
+        dg = ctx.dg
+        g = g.contiguous()
+        grad_psi = grad_v = None
+        if ctx.needs_input_grad[0]:
+            grad_psi = apply_operator(dg, g, scale=-0.5, closure=ctx.closure,
+                                      veff=veff.detach().reshape(dg.shape), transpose=True)
+        if ctx.needs_input_grad[1]:
+            prod = (psi.detach() * g).reshape(-1, dg.N).sum(0)
+            grad_v = prod.reshape(veff.shape)
+        return grad_psi, grad_v, None, None
+
+
+def hpsi(psi, veff, dg, closure="production"):
+    return HPsi.apply(psi, veff, dg, closure)
+
+
+class RitzValues(torch.autograd.Function):
+    """eps_1..k of T + V from a starting block; backward: w * sum_i g_i psi_i^2."""
+
+    @staticmethod
+    def forward(ctx, veff, dg: DeviceGrid, X0, k, tol=1e-8, closure="trial_fix", max_iter=400):
+        from .eigensolver import block_subspace_iteration
+        from .hamiltonian import BlockHamiltonian
+
+        H = BlockHamiltonian(dg, veff.detach().reshape(-1), closure=closure)
+        res = block_subspace_iteration(H, dg.weights(), X0, k, tol, max_iter)
+        if not res.converged:
+            raise RuntimeError("the block eigensolver did not converge inside the autograd forward")
+        psi = res.orbitals.reshape(k, -1).to(torch.float64)
+        ctx.save_for_backward(psi)
+        ctx.dg = dg
+        ctx.vshape = veff.shape
+        return torch.as_tensor(np.asarray(res.eigenvalues), dtype=torch.float64, device=veff.device)
+
+    @staticmethod
+    def backward(ctx, g):
+        (psi,) = ctx.saved_tensors
+        w = ctx.dg.weights()
+        grad = w * (g.to(psi.device)[:, None] * psi * psi).sum(0)
+        return grad.reshape(ctx.vshape), None, None, None, None, None, None
+
+
+def ritz_values(veff, dg, X0, k, tol=1e-8, closure="trial_fix"):
+    return RitzValues.apply(veff, dg, X0, k, tol, closure)
+
+
+def _ion_ion(elements, R):
+    e = 0.0
+    g = np.zeros_like(R)
+    for i in range(len(elements)):
+        for j in range(i + 1, len(elements)):
+            zi = pseudo.GTH_PADE[elements[i]].ionic_charge
+            zj = pseudo.GTH_PADE[elements[j]].ionic_charge
+            d = R[i] - R[j]
+            r = float(np.linalg.norm(d))
+            e += zi * zj / r
+            g[i] -= zi * zj * d / r ** 3
+            g[j] += zi * zj * d / r ** 3
+    return e, g
+
+
+class IonicEnergy(torch.autograd.Function):
+    """E_loc + E_nl + E_II at frozen rho / orbitals as a function of positions (natoms, 3)."""
+
+    @staticmethod
+    def forward(ctx, positions, dg: DeviceGrid, elements, rho, orbitals, occupations):
+        R = positions.detach().to("cpu", torch.float64).numpy()
+        atoms = [(el, tuple(float(c) for c in r)) for el, r in zip(elements, R)]
+        rho = rho.reshape(-1).to(torch.float64)
+        vloc = pseudo.local_potential_device(dg, atoms)
+        e_loc = wdot(dg, rho, vloc)
+        plan = pseudo.NonlocalPlan(dg, atoms)
+        occ = np.asarray(occupations, dtype=np.float64)
+        X = orbitals.reshape(-1, dg.N)[:len(occ)].to(torch.float64).contiguous()
+        e_nl = float(np.dot(occ, plan.energy_terms(X))) if not plan.empty else 0.0
+        e_ii, g_ii = _ion_ion(list(elements), R)
+        ctx.dg, ctx.atoms, ctx.elements = dg, atoms, list(elements)
+        ctx.plan, ctx.occ, ctx.g_ii = plan, occ, g_ii
+        ctx.save_for_backward(rho, X)
+        ctx.pdevice = positions.device
+        return torch.tensor(e_loc + e_nl + e_ii, dtype=torch.float64, device=positions.device)
+
+    @staticmethod
+    def backward(ctx, g):
+        rho, X = ctx.saved_tensors
+        dg = ctx.dg
+        grad = pseudo.local_potential_gradient_device(dg, ctx.atoms, rho * dg.weights()) + ctx.g_ii
+        plan = ctx.plan
+        if not plan.empty:
+            nx, ny, nz = dg.shape
+            from . import _lib
+            from .reductions import workspace
+
+            lib = _lib.load()
+            k = X.shape[0]
+            for ap, coef, d in plan.project(X):
+                for axis in range(3):
+                    dP = ap.field_derivative(dg, axis)
+                    work = workspace(dg.device, lib.ig_nonlocal_work_size(k, ap.nproj), "nl")
+                    dc = torch.empty((k, ap.nproj), dtype=torch.float64, device=dg.device)
+                    dd = torch.empty_like(dc)
+                    _lib.call("ig_nonlocal_project_f64", _lib.ptr(X), X.stride(0), k, nx, ny, nz, ap.box,
+                              _lib.ptr(dP), ap.nproj, _lib.ptr(dg.weights()), _lib.ptr(ap.H), _lib.ptr(work),
+                              _lib.ptr(dc), _lib.ptr(dd), _lib.stream_handle())
+                    # d/dR <psi|p> h <p|psi> = 2 <psi|dp> h <p|psi>  (h symmetric)
+                    per = 2.0 * (dc * d).sum(dim=1).cpu().numpy()
+                    grad[ap.atom_index, axis] += float(np.dot(ctx.occ, per))
+        gt = torch.as_tensor(grad, dtype=torch.float64, device=ctx.pdevice) * g
+        return gt, None, None, None, None, None
+
+
+def ionic_energy(positions, dg, elements, rho, orbitals, occupations):
+    return IonicEnergy.apply(positions, dg, elements, rho, orbitals, occupations)
+
+
+def atomic_forces(dg, elements, positions, rho, orbitals, occupations):
+    """Hellmann-Feynman forces -dE_ion/dR (natoms, 3) at frozen rho and orbitals."""
+    R = torch.as_tensor(np.asarray(positions, dtype=np.float64)).requires_grad_(True)
+    e = ionic_energy(R, dg, elements, rho, orbitals, occupations)
+    (g,) = torch.autograd.grad(e, R)
+    return -g.numpy()
