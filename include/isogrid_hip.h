@@ -141,6 +141,11 @@ int ig_vloc_gth_grad_f64(const double* xyz, const double* q, long long N, int na
  * libxc 'lda,vwn', this kernel evaluates PW92). */
 int ig_lsda_pw92_f64(const double* rho_up, const double* rho_down, long long N, double* eps,
                      double* v_up, double* v_down, int* flag, hipStream_t stream);
+/* The same with VWN5 correlation (the functional behind the reference's libxc 'lda,vwn',
+ * xc/lsda.py:22, :95-100): Vosko-Wilk-Nusair 1980 paramagnetic / ferromagnetic / spin-
+ * stiffness fits with the VWN spin interpolation.  Replaces: xc/lsda.py:70-129. */
+int ig_lsda_vwn_f64(const double* rho_up, const double* rho_down, long long N, double* eps,
+                    double* v_up, double* v_down, int* flag, hipStream_t stream);
 
 /* ------------------------------------------------------------------ nonlocal projectors */
 

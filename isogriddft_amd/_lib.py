@@ -41,6 +41,7 @@ SIGNATURES = {
     "ig_vloc_gth_f64": [P, L, I, P, P, I, P],
     "ig_vloc_gth_grad_f64": [P, P, L, I, P, P, P, P],
     "ig_lsda_pw92_f64": [P, P, L, P, P, P, P, P],
+    "ig_lsda_vwn_f64": [P, P, L, P, P, P, P, P],
     "ig_projector_desc_size": [],
     "ig_projector_fields_f64": [P, I, I, I, P, P, I, P, P, I, P],
     "ig_nonlocal_work_size": [I, I],

@@ -140,6 +140,33 @@ __device__ __forceinline__ void pw_g(double rs, const Pw& p, double& g, double& 
   dg = -2.0 * p.A * p.a1 * lg - q0 * q1p / (q1 * q1 + q1);
 }
 
+// VWN interpolation function (Vosko, Wilk, Nusair 1980, "VWN5" fit) of x = sqrt(rs) and
+// its x derivative for the parameter set (A, b, c, x0).
+struct Vwn {
+  double A, b, c, x0;
+};
+__constant__ Vwn c_vwn[3] = {
+    {0.0310907, 3.72744, 12.9352, -0.10498},                        // paramagnetic
+    {0.01554535, 7.06042, 18.0578, -0.32500},                       // ferromagnetic
+    {-1.0 / (6.0 * PI * PI), 1.13107, 13.0045, -0.0047584},         // spin stiffness
+};
+
+__device__ __forceinline__ void vwn_g(double x, const Vwn& p, double& g, double& dg) {
+  const double X = x * x + p.b * x + p.c;
+  const double X0 = p.x0 * p.x0 + p.b * p.x0 + p.c;
+  const double Q = sqrt(4.0 * p.c - p.b * p.b);
+  const double t = 2.0 * x + p.b;
+  const double at = atan(Q / t);
+  const double k = p.b * p.x0 / X0;
+  g = p.A * (log(x * x / X) + 2.0 * p.b / Q * at - k * (log((x - p.x0) * (x - p.x0) / X) + 2.0 * (p.b + 2.0 * p.x0) / Q * at));
+  const double den = t * t + Q * Q;
+  dg = p.A * (2.0 / x - t / X - 4.0 * p.b / den - k * (2.0 / (x - p.x0) - t / X - 4.0 * (p.b + 2.0 * p.x0) / den));
+}
+
+// Spin-polarised Slater exchange + correlation per point.  FUNC 0: Perdew-Wang 1992;
+// FUNC 1: VWN5 with the VWN spin interpolation
+//   eps_c = eps_P + alpha f(z) / f''(0) (1 - z^4) + (eps_F - eps_P) f(z) z^4.
+template <int FUNC>
 __global__ __launch_bounds__(IG_THREADS) void lsda_kernel(const double* __restrict__ ru, const double* __restrict__ rd,
                                                           long long N, double* __restrict__ eps,
                                                           double* __restrict__ vu, double* __restrict__ vd,
@@ -160,18 +187,33 @@ __global__ __launch_bounds__(IG_THREADS) void lsda_kernel(const double* __restri
       zeta = fmin(1.0, fmax(-1.0, zeta));
       const double exv = -0.75 * cx * (a * cbrt(a) + b * cbrt(b));
       const double vxu = -cx * cbrt(a), vxd = -cx * cbrt(b);
-      double g0, d0, g1, d1, g2, d2;
-      pw_g(rs, c_pw[0], g0, d0);
-      pw_g(rs, c_pw[1], g1, d1);
-      pw_g(rs, c_pw[2], g2, d2);
-      const double ac = -g2, dac = -d2;
+      double g0, d0, g1, d1, g2, d2, ac, dac, fpp;
+      if (FUNC == 0) {
+        pw_g(rs, c_pw[0], g0, d0);
+        pw_g(rs, c_pw[1], g1, d1);
+        pw_g(rs, c_pw[2], g2, d2);
+        ac = -g2;
+        dac = -d2;
+        fpp = fz20;
+      } else {
+        // x = sqrt(rs): d/drs = (d/dx) / (2x)
+        const double x = sqrt(rs);
+        vwn_g(x, c_vwn[0], g0, d0);
+        vwn_g(x, c_vwn[1], g1, d1);
+        vwn_g(x, c_vwn[2], ac, dac);
+        const double ix = 0.5 / x;
+        d0 *= ix;
+        d1 *= ix;
+        dac *= ix;
+        fpp = 1.7099209341613657;  // 4 / (9 (2^(1/3) - 1))
+      }
       const double opz = cbrt(1.0 + zeta), omz = cbrt(1.0 - zeta);
       const double fz = ((1.0 + zeta) * opz + (1.0 - zeta) * omz - 2.0) / fzden;
       const double dfz = (4.0 / 3.0) * (opz - omz) / fzden;
       const double z3 = zeta * zeta * zeta, z4 = z3 * zeta;
-      const double ec = g0 + ac * fz / fz20 * (1.0 - z4) + (g1 - g0) * fz * z4;
-      const double decdrs = d0 + dac * fz / fz20 * (1.0 - z4) + (d1 - d0) * fz * z4;
-      const double decdz = ac / fz20 * (dfz * (1.0 - z4) - 4.0 * z3 * fz) + (g1 - g0) * (dfz * z4 + 4.0 * z3 * fz);
+      const double ec = g0 + ac * fz / fpp * (1.0 - z4) + (g1 - g0) * fz * z4;
+      const double decdrs = d0 + dac * fz / fpp * (1.0 - z4) + (d1 - d0) * fz * z4;
+      const double decdz = ac / fpp * (dfz * (1.0 - z4) - 4.0 * z3 * fz) + (g1 - g0) * (dfz * z4 + 4.0 * z3 * fz);
       const double vcc = ec - rs / 3.0 * decdrs;
       e = exv / rho + ec;
       pu = vxu + vcc - (zeta - 1.0) * decdz;
@@ -595,7 +637,16 @@ int ig_vloc_gth_grad_f64(const double* xyz, const double* q, long long N, int na
 int ig_lsda_pw92_f64(const double* rho_up, const double* rho_down, long long N, double* eps, double* v_up,
                      double* v_down, int* flag, hipStream_t s) {
   if (!rho_up || !rho_down || !eps || !v_up || !v_down || !flag || N <= 0) return IG_EINVAL;
-  hipLaunchKernelGGL(lsda_kernel, dim3(grid1d(N)), dim3(IG_THREADS), 0, s, rho_up, rho_down, N, eps, v_up, v_down,
+  hipLaunchKernelGGL(lsda_kernel<0>, dim3(grid1d(N)), dim3(IG_THREADS), 0, s, rho_up, rho_down, N, eps, v_up, v_down,
+                     flag);
+  IG_CHECK_LAUNCH();
+  return IG_OK;
+}
+
+int ig_lsda_vwn_f64(const double* rho_up, const double* rho_down, long long N, double* eps, double* v_up,
+                     double* v_down, int* flag, hipStream_t s) {
+  if (!rho_up || !rho_down || !eps || !v_up || !v_down || !flag || N <= 0) return IG_EINVAL;
+  hipLaunchKernelGGL(lsda_kernel<1>, dim3(grid1d(N)), dim3(IG_THREADS), 0, s, rho_up, rho_down, N, eps, v_up, v_down,
                      flag);
   IG_CHECK_LAUNCH();
   return IG_OK;

@@ -5,7 +5,7 @@ scf/driver.py:630-690 and :884-1015).
   E_loc = int rho V_loc                    ig_vloc_gth_f64 + FP64 weighted dot
   E_nl  = sum_occ <psi|V_nl psi>_W         projections on the support boxes
   E_H   = 1/2 int rho v_H                  open-boundary Poisson solve
-  E_xc  = int rho eps_xc                   ig_lsda_pw92_f64 (Slater + PW92)
+  E_xc  = int rho eps_xc                   ig_lsda_vwn_f64 (default) / ig_lsda_pw92_f64
   E_II  = sum_{i<j} Z_i Z_j / R_ij         host (a few atoms)
 Densities sum_m occ_m psi_m^2 come from ig_density_*.
 """
@@ -134,8 +134,7 @@ def energy_components_device(dg, orbitals_up, orbitals_down, occupations, case, 
     sol = poisson.solve_poisson_device(dg, rho.clamp_min(0.0), tolerance=poisson_tolerance,
                                        precision=poisson_precision)
     eh = 0.5 * wdot(dg, rho, sol["potential"].reshape(-1))
-    xc.normalize_functional(functional)
-    eps, _, _ = xc.lsda_device(ru.view(dg.shape), rd.view(dg.shape))
+    eps, _, _ = xc.lsda_device(ru.view(dg.shape), rd.view(dg.shape), functional=functional)
     exc = wdot(dg, rho, eps.reshape(-1))
     eii = evaluate_ion_ion_repulsion(case)
     total = kin + eloc + enl + eh + exc + eii

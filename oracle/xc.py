@@ -91,3 +91,64 @@ def lsda_pw92(rho_up, rho_down):
 def lsda_energy_density(rho_up, rho_down):
     eps, _, _ = lsda_pw92(rho_up, rho_down)
     return (validate_density(rho_up) + validate_density(rho_down)) * eps
+
+
+# ---------------------------------------------------------------- VWN (libxc 'lda,vwn')
+# Slater exchange + Vosko-Wilk-Nusair correlation, parameterisation "VWN5"
+# (S. H. Vosko, L. Wilk, M. Nusair, Can. J. Phys. 58, 1200 (1980), fit to the Ceperley-
+# Alder data; the functional behind PySCF's 'lda,vwn', xc/lsda.py:22, :95-100 of the
+# reference).  Parameter sets (A, b, c, x0) for the paramagnetic and ferromagnetic
+# correlation energies and the spin stiffness; x = sqrt(rs).
+VWN_P = (0.0310907, 3.72744, 12.9352, -0.10498)
+VWN_F = (0.01554535, 7.06042, 18.0578, -0.32500)
+VWN_A = (-1.0 / (6.0 * np.pi ** 2), 1.13107, 13.0045, -0.0047584)
+VWN_FPP = 4.0 / (9.0 * (2.0 ** (1.0 / 3.0) - 1.0))
+
+
+def _vwn_g(x, p):
+    """VWN interpolation function and its derivative with respect to x."""
+    A, b, c, x0 = p
+    X = x * x + b * x + c
+    X0 = x0 * x0 + b * x0 + c
+    Q = np.sqrt(4.0 * c - b * b)
+    at = np.arctan(Q / (2.0 * x + b))
+    g = A * (np.log(x * x / X) + 2.0 * b / Q * at
+             - b * x0 / X0 * (np.log((x - x0) ** 2 / X) + 2.0 * (b + 2.0 * x0) / Q * at))
+    den = (2.0 * x + b) ** 2 + Q * Q
+    dg = A * (2.0 / x - (2.0 * x + b) / X - 4.0 * b / den
+              - b * x0 / X0 * (2.0 / (x - x0) - (2.0 * x + b) / X - 4.0 * (b + 2.0 * x0) / den))
+    return g, dg
+
+
+def lsda_vwn(rho_up, rho_down):
+    """(eps_xc, v_up, v_down) of Slater + VWN5 with the VWN spin interpolation
+    eps_c = eps_P + alpha f(z)/f''(0) (1 - z^4) + (eps_F - eps_P) f(z) z^4."""
+    a = validate_density(rho_up, "rho_up")
+    b = validate_density(rho_down, "rho_down")
+    rho = a + b
+    ok = rho > DENSITY_FLOOR
+    rs = np.where(ok, (3.0 / (4.0 * np.pi * np.where(ok, rho, 1.0))) ** (1.0 / 3.0), 1.0)
+    z = np.clip(np.where(ok, (a - b) / np.where(ok, rho, 1.0), 0.0), -1.0, 1.0)
+    cx = (6.0 / np.pi) ** (1.0 / 3.0)
+    ex_vol = -0.75 * cx * (a ** (4.0 / 3.0) + b ** (4.0 / 3.0))
+    vx_up = -cx * np.cbrt(a)
+    vx_dn = -cx * np.cbrt(b)
+    x = np.sqrt(rs)
+    gp, dp = _vwn_g(x, VWN_P)
+    gf, df = _vwn_g(x, VWN_F)
+    ga, da = _vwn_g(x, VWN_A)
+    opz, omz = np.cbrt(1.0 + z), np.cbrt(1.0 - z)
+    fz = ((1.0 + z) * opz + (1.0 - z) * omz - 2.0) / FZ_DEN
+    dfz = (4.0 / 3.0) * (opz - omz) / FZ_DEN
+    z3 = z ** 3
+    z4 = z * z3
+    ec = gp + ga * fz / VWN_FPP * (1.0 - z4) + (gf - gp) * fz * z4
+    # d/drs = (d/dx) / (2 x)
+    decdx = dp + da * fz / VWN_FPP * (1.0 - z4) + (df - dp) * fz * z4
+    decdrs = decdx / (2.0 * x)
+    decdz = ga / VWN_FPP * (dfz * (1.0 - z4) - 4.0 * z3 * fz) + (gf - gp) * (dfz * z4 + 4.0 * z3 * fz)
+    vc_common = ec - rs / 3.0 * decdrs
+    eps = np.where(ok, ex_vol / np.where(ok, rho, 1.0) + ec, 0.0)
+    v_up = np.where(ok, vx_up + vc_common - (z - 1.0) * decdz, 0.0)
+    v_dn = np.where(ok, vx_dn + vc_common - (z + 1.0) * decdz, 0.0)
+    return eps, v_up, v_dn

@@ -65,7 +65,7 @@ def test_single_point_components_match_oracle():
         enl = sum(oenergy.expectation(p, opseudo.nonlocal_action(p, case, geom), w) for p in np.concatenate([up, dn]))
         pot, _, _, _ = opoisson.solve_poisson(geom, rho, order=2, method="direct")
         eh = opoisson.hartree_energy(rho, geom, pot)
-        eps, _, _ = oxc.lsda_pw92(ru, rd)
+        eps, _, _ = oxc.lsda_vwn(ru, rd)  # the default functional, as the reference
         exc = float(np.sum(rho * eps * w))
         eii = oenergy.ion_ion(case)
         ref = dict(kinetic=kin, local_ionic=eloc, nonlocal_ionic=enl, hartree=eh, xc=exc, ion_ion_repulsion=eii)

@@ -150,7 +150,8 @@ def test_nonlocal_call_surface():
     assert _rel(s, ref) <= TOL
 
 
-def test_lsda_matches_oracle_and_rejects_negative_density():
+@pytest.mark.parametrize("functional", ["lda,vwn", "lda,pw92"])
+def test_lsda_matches_oracle_and_rejects_negative_density(functional):
     rng = np.random.default_rng(11)
     shape = (20, 21, 22)
     up = rng.uniform(0, 2, shape) ** 3
@@ -159,16 +160,22 @@ def test_lsda_matches_oracle_and_rejects_negative_density():
     dn[1, 1, :4] = 0.0
     up[2, 2, 0] = dn[2, 2, 0] = 0.0
     up[3, 3, 0] = -5e-15
-    ev = xc.evaluate_lsda_terms(up, dn)
-    eps, vu, vd = oxc.lsda_pw92(up, dn)
+    ev = xc.evaluate_lsda_terms(up, dn, functional=functional)
+    ref = oxc.lsda_vwn if functional == "lda,vwn" else oxc.lsda_pw92
+    eps, vu, vd = ref(up, dn)
     assert _rel(ev.eps_xc, eps) <= TOL
     assert _rel(ev.v_xc_up, vu) <= TOL
     assert _rel(ev.v_xc_down, vd) <= TOL
     with pytest.raises(ValueError):
-        xc.evaluate_lsda_terms(up - 1e-3, dn)
-    with pytest.raises(ValueError):
-        xc.evaluate_lsda_terms(up, dn, functional="lda,vwn")
+        xc.evaluate_lsda_terms(up - 1e-3, dn, functional=functional)
     bad = dn.copy()
     bad[0, 0, 0] = np.nan
     with pytest.raises(ValueError):
-        xc.evaluate_lsda_terms(up, bad)
+        xc.evaluate_lsda_terms(up, bad, functional=functional)
+
+
+def test_lsda_default_functional_and_unknown_names():
+    up = np.full((4, 4, 4), 0.1)
+    assert xc.evaluate_lsda_terms(up, up).functional == "lda,vwn"
+    with pytest.raises(ValueError):
+        xc.evaluate_lsda_terms(up, up, functional="gga,pbe")

@@ -3,7 +3,9 @@
 Reference values (src/isogrid/audit/baselines.py of the reference, bonding trial orbital
 psi = normalised sum_A exp(-0.8 r_A^2) on the 51^3 legacy grid, (8, 8, 10) Bohr, stretch
 (2.5, 2.5, 2.8)):  <T> = 0.997532314836 (:1066), <psi|V_loc|psi> = -2.173678262801
-(:1067), E_H = 1.748670071699 Ha (:1068, rho = 2 psi^2, BiCGSTAB rtol 1e-8).
+(:1067), E_H = 1.748670071699 Ha (:1068, rho = 2 psi^2, BiCGSTAB rtol 1e-8),
+E_xc = -0.869107874980 (:822) and <psi|v_xc|psi> = -0.568636849315 (:1069) for
+rho_up = rho_down = psi^2 with libxc 'lda,vwn'.
 Bars: the recorded digits (1e-11 Ha for <T>, <V_loc>; 1e-8 Ha for E_H, whose recorded value
 carries the reference's own 1e-8 solver tolerance).
 """
@@ -51,3 +53,16 @@ def test_hartree_energy(legacy51, precision):
     assert res.converged
     eh = poisson.evaluate_hartree_energy(rho, g, res)
     assert abs(eh - 1.748670071699) < 1e-8, (precision, eh)
+
+
+def test_xc_energy_and_potential_expectation(legacy51):
+    from isogriddft_amd import xc
+
+    g = legacy51
+    psi = _trial(g)
+    r = psi * psi
+    exc = xc.evaluate_lsda_energy(r, r, g)
+    assert abs(exc - (-0.869107874980)) < 1e-11, exc
+    ev = xc.evaluate_lsda_terms(r, r)
+    vexp = kinetic.integrate_field(psi * psi * ev.v_xc_up, g)
+    assert abs(vexp - (-0.568636849315)) < 1e-11, vexp

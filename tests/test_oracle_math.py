@@ -178,3 +178,29 @@ def test_block_iteration_matches_dense_eigh():
     assert ok
     np.testing.assert_allclose(ev, ev_dense, rtol=0, atol=1e-10)
     np.testing.assert_allclose(eigen.gram(Psi, Psi, w), np.eye(3), atol=1e-10)
+
+
+def test_vwn_potentials_are_energy_derivatives_and_limits():
+    """Slater + VWN5: v_sigma = d(rho eps)/d rho_sigma (central differences, polarised
+    densities), eps at z = 1 is the ferromagnetic fit, eps at z = 0 the paramagnetic one."""
+    rng = np.random.default_rng(4)
+    a = rng.uniform(1e-4, 2.0, 200)
+    b = rng.uniform(1e-4, 2.0, 200)
+    _, vu, vd = xc.lsda_vwn(a, b)
+    h = 1e-6
+    e = lambda x, y: (x + y) * xc.lsda_vwn(x, y)[0]  # noqa: E731
+    du = (e(a + h * a, b) - e(a - h * a, b)) / (2 * h * a)
+    dd = (e(a, b + h * b) - e(a, b - h * b)) / (2 * h * b)
+    np.testing.assert_allclose(vu, du, rtol=2e-7, atol=1e-9)
+    np.testing.assert_allclose(vd, dd, rtol=2e-7, atol=1e-9)
+    rho = np.array([0.01, 0.3, 2.0])
+    rs = (3.0 / (4.0 * np.pi * rho)) ** (1.0 / 3.0)
+    x = np.sqrt(rs)
+    ex_p = -0.75 * (3.0 / np.pi) ** (1.0 / 3.0) * np.cbrt(rho)
+    ex_f = 2.0 ** (1.0 / 3.0) * ex_p
+    eps0, _, _ = xc.lsda_vwn(rho / 2, rho / 2)
+    np.testing.assert_allclose(eps0, ex_p + xc._vwn_g(x, xc.VWN_P)[0], rtol=1e-13)
+    eps1, _, _ = xc.lsda_vwn(rho, 0 * rho)
+    np.testing.assert_allclose(eps1, ex_f + xc._vwn_g(x, xc.VWN_F)[0], rtol=1e-13)
+    # the paramagnetic fit reproduces the VWN paper's ec(rs = 1) ~ -0.0600 Ha (Ceperley-Alder)
+    assert abs(xc._vwn_g(np.array([1.0]), xc.VWN_P)[0][0] - (-0.0600)) < 5e-4

@@ -2,6 +2,8 @@
 
 Reference-held scalars (src/isogrid/audit/baselines.py of the reference):
   legacy 51^3 H2 grid, bonding trial orbital:  <T> = 0.997532314836 (:1066),
+      E_xc[rho_up = rho_down = psi^2] = -0.869107874980 (:822), <psi|v_xc|psi> =
+      -0.568636849315 (:1069), both with PySCF/libxc 'lda,vwn',
       <psi|V_loc|psi> = -2.173678262801 (:1067), E_loc = -4.347356525602 (:830),
       E_H = 1.748670071699 (:1068, SciPy BiCGSTAB rtol 1e-8)
   monitor 67x67x81 H2 development grid:        <T> = 0.9671226925733339 (:1088, :1110,
@@ -38,6 +40,20 @@ def test_legacy_kinetic_and_local_scalars(legacy51):
     V = pseudo.local_ionic_potential(grid.H2_CASE, g)
     assert abs(float(np.sum(psi * psi * V * w)) - (-2.173678262801)) < 1e-11
     assert abs(float(np.sum(2 * psi * psi * V * w)) - (-4.347356525602)) < 1e-11
+
+
+def test_legacy_xc_scalars_pin_vwn(legacy51):
+    """The VWN5 restatement reproduces the reference's libxc 'lda,vwn' values."""
+    from oracle import xc
+
+    g = legacy51
+    psi = _trial(g)
+    r = psi * psi
+    eps, vu, vd = xc.lsda_vwn(r, r)
+    w = g.cell_volumes
+    assert abs(float(np.sum(2 * r * eps * w)) - (-0.869107874980)) < 1e-11
+    assert abs(float(np.sum(psi * psi * vu * w)) - (-0.568636849315)) < 1e-11
+    assert np.array_equal(vu, vd)
 
 
 def test_legacy_hartree_energy(legacy51):
