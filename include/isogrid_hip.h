@@ -219,6 +219,16 @@ int ig_cell_moments_f64(const double* xyz, const double* jac, int nx, int ny, in
                         const int* cells, int ncells, const double* coef, double subvol,
                         const double* center, int order, double* work, double* out,
                         hipStream_t stream);
+/* Split correction of the monitor `legacy_split` boundary mode: with
+ * d = (rho_fit - rho_trilinear) J_trilinear subvol at the 2x2x2 sub-cell midpoints of the
+ * selected cells, center == NULL gives out[4] = [sum |d|, sum |d| x, sum |d| y,
+ * sum |d| z] (the correction centre), otherwise the raw moments of d about center
+ * (host pointer) up to order.  Replaces: poisson/open_boundary.py:559-719
+ * (_monitor_grid_boundary_value_correction). */
+int ig_cell_split_moments_f64(const double* rho, const double* xyz, const double* jac, int nx, int ny,
+                              int nz, const int* cells, int ncells, const double* coef, double subvol,
+                              const double* center, int order, double* work, double* out,
+                              hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -55,6 +55,7 @@ SIGNATURES = {
     "ig_boundary_values_f64": [P, I, I, I, P, I, P, P, P],
     "ig_cell_fit_f64": [P, P, I, I, I, P, I, P, P],
     "ig_cell_moments_f64": [P, P, I, I, I, P, I, P, D, P, I, P, P, P],
+    "ig_cell_split_moments_f64": [P, P, P, I, I, I, P, I, P, D, P, I, P, P, P],
 }
 _RESTYPE = {"ig_gram_work_size": L}
 

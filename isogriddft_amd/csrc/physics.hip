@@ -515,14 +515,18 @@ __global__ __launch_bounds__(IG_THREADS) void cell_fit_kernel(const double* __re
   }
 }
 
-// Moments of the quadratic fit sampled at the 2x2x2 sub-cell midpoints (trilinear
-// position and Jacobian), weight rho_fit * J * subvol.
-template <int ORDER>
+// Sums over the 2x2x2 sub-cell midpoints of the selected cells (trilinear position and
+// Jacobian).  MODE 0: raw moments about c of q = rho_fit J subvol (corrected moments);
+// MODE 1: [sum |d|, sum |d| x, sum |d| y, sum |d| z] and MODE 2: raw moments about c of
+// d = (rho_fit - rho_trilinear) J subvol (the split correction of the reference's
+// legacy_split mode, poisson/open_boundary.py:559-719).
+template <int ORDER, int MODE>
 __global__ __launch_bounds__(IG_THREADS) void cell_moments_kernel(const double* __restrict__ xyz, const double* __restrict__ jac,
+                                                                  const double* __restrict__ rho,
                                                                   int nx, int ny, int nz, const int* __restrict__ cells,
                                                                   int ncells, const double* __restrict__ coef, double subvol,
                                                                   double cx, double cy, double cz, double* __restrict__ partial) {
-  constexpr int NM = nmono(ORDER);
+  constexpr int NM = MODE == 1 ? 4 : nmono(ORDER);
   __shared__ double smem[4 * NM];
   const long long N = (long long)nx * ny * nz;
   double acc[NM];
@@ -530,7 +534,7 @@ __global__ __launch_bounds__(IG_THREADS) void cell_moments_kernel(const double* 
   for (int i = 0; i < NM; ++i) acc[i] = 0.0;
   for (int c = blockIdx.x * IG_THREADS + threadIdx.x; c < ncells; c += gridDim.x * IG_THREADS) {
     const int ci = cells[3 * c], cj = cells[3 * c + 1], ck = cells[3 * c + 2];
-    double X[8], Y[8], Z[8], Jc[8];
+    double X[8], Y[8], Z[8], Jc[8], Rc[8];
     for (int a = 0; a < 2; ++a)
       for (int b = 0; b < 2; ++b)
         for (int d = 0; d < 2; ++d) {
@@ -540,6 +544,7 @@ __global__ __launch_bounds__(IG_THREADS) void cell_moments_kernel(const double* 
           Y[e] = xyz[N + g];
           Z[e] = xyz[2 * N + g];
           Jc[e] = jac[g];
+          Rc[e] = MODE == 0 ? 0.0 : rho[g];
         }
     const double* cf = coef + (long long)c * 14;
     const double ox = cf[10], oy = cf[11], oz = cf[12], is = cf[13];
@@ -550,22 +555,31 @@ __global__ __launch_bounds__(IG_THREADS) void cell_moments_kernel(const double* 
         const double fu = (e & 4) ? u : 1.0 - u, fv = (e & 2) ? v : 1.0 - v, fw = (e & 1) ? ww : 1.0 - ww;
         tri[e] = fu * fv * fw;
       }
-      double xs = 0.0, ys = 0.0, zs = 0.0, js = 0.0;
+      double xs = 0.0, ys = 0.0, zs = 0.0, js = 0.0, rt = 0.0;
       for (int e = 0; e < 8; ++e) {
         xs += tri[e] * X[e];
         ys += tri[e] * Y[e];
         zs += tri[e] * Z[e];
         js += tri[e] * Jc[e];
+        rt += tri[e] * Rc[e];
       }
       double m2[10];
       monomials<2>((xs - ox) * is, (ys - oy) * is, (zs - oz) * is, m2);
       double fit = 0.0;
       for (int r = 0; r < 10; ++r) fit += cf[r] * m2[r];
-      const double q = fit * js * subvol;
-      double mm[NM];
-      monomials<ORDER>(xs - cx, ys - cy, zs - cz, mm);
+      const double q = (MODE == 0 ? fit : fit - rt) * js * subvol;
+      if constexpr (MODE == 1) {
+        const double aq = fabs(q);
+        acc[0] += aq;
+        acc[1] += aq * xs;
+        acc[2] += aq * ys;
+        acc[3] += aq * zs;
+      } else {
+        double mm[nmono(ORDER)];
+        monomials<ORDER>(xs - cx, ys - cy, zs - cz, mm);
 #pragma unroll
-      for (int i = 0; i < NM; ++i) acc[i] += q * mm[i];
+        for (int i = 0; i < nmono(ORDER); ++i) acc[i] += q * mm[i];
+      }
     }
   }
   ig_block_sum<NM>(acc, smem);
@@ -587,15 +601,17 @@ int moments_t(const double* rho, const double* w, const double* frac, const doub
   return IG_OK;
 }
 
-template <int ORDER>
+template <int ORDER, int MODE = 0>
 int cell_moments_t(const double* xyz, const double* jac, int nx, int ny, int nz, const int* cells, int ncells,
-                   const double* coef, double subvol, const double* center, double* work, double* out, hipStream_t s) {
-  constexpr int NM = nmono(ORDER);
+                   const double* coef, double subvol, const double* center, double* work, double* out, hipStream_t s,
+                   const double* rho = nullptr) {
+  constexpr int NM = MODE == 1 ? 4 : nmono(ORDER);
   int nb = (ncells + IG_THREADS - 1) / IG_THREADS;
   if (nb > 1024) nb = 1024;
   if (nb < 1) nb = 1;
-  hipLaunchKernelGGL(cell_moments_kernel<ORDER>, dim3(nb), dim3(IG_THREADS), 0, s, xyz, jac, nx, ny, nz, cells, ncells,
-                     coef, subvol, center[0], center[1], center[2], work);
+  const double c0 = center ? center[0] : 0.0, c1 = center ? center[1] : 0.0, c2 = center ? center[2] : 0.0;
+  hipLaunchKernelGGL((cell_moments_kernel<ORDER, MODE>), dim3(nb), dim3(IG_THREADS), 0, s, xyz, jac, rho, nx, ny, nz,
+                     cells, ncells, coef, subvol, c0, c1, c2, work);
   hipLaunchKernelGGL(sum_partials, dim3(1), dim3(64), 0, s, work, nb, NM, out);
   IG_CHECK_LAUNCH();
   return IG_OK;
@@ -773,6 +789,21 @@ int ig_cell_moments_f64(const double* xyz, const double* jac, int nx, int ny, in
     case 2: return cell_moments_t<2>(xyz, jac, nx, ny, nz, cells, ncells, coef, subvol, center, work, out, s);
     case 3: return cell_moments_t<3>(xyz, jac, nx, ny, nz, cells, ncells, coef, subvol, center, work, out, s);
     case 4: return cell_moments_t<4>(xyz, jac, nx, ny, nz, cells, ncells, coef, subvol, center, work, out, s);
+    default: return IG_EUNSUPPORTED;
+  }
+}
+
+int ig_cell_split_moments_f64(const double* rho, const double* xyz, const double* jac, int nx, int ny, int nz,
+                              const int* cells, int ncells, const double* coef, double subvol, const double* center,
+                              int order, double* work, double* out, hipStream_t s) {
+  if (!rho || !xyz || !jac || !cells || !coef || !work || !out || ncells <= 0) return IG_EINVAL;
+  if (!center) return cell_moments_t<0, 1>(xyz, jac, nx, ny, nz, cells, ncells, coef, subvol, nullptr, work, out, s, rho);
+  switch (order) {
+    case 0: return cell_moments_t<0, 2>(xyz, jac, nx, ny, nz, cells, ncells, coef, subvol, center, work, out, s, rho);
+    case 1: return cell_moments_t<1, 2>(xyz, jac, nx, ny, nz, cells, ncells, coef, subvol, center, work, out, s, rho);
+    case 2: return cell_moments_t<2, 2>(xyz, jac, nx, ny, nz, cells, ncells, coef, subvol, center, work, out, s, rho);
+    case 3: return cell_moments_t<3, 2>(xyz, jac, nx, ny, nz, cells, ncells, coef, subvol, center, work, out, s, rho);
+    case 4: return cell_moments_t<4, 2>(xyz, jac, nx, ny, nz, cells, ncells, coef, subvol, center, work, out, s, rho);
     default: return IG_EUNSUPPORTED;
   }
 }

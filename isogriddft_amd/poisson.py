@@ -286,15 +286,22 @@ def _center_arg(center):
     return ctypes.cast(arr, ctypes.c_void_p), arr
 
 
+def _cell_fit(dg, pdata, rho, stream=None):
+    """Per selected cell: the quadratic least-squares fit over its 4x4x4 stencil."""
+    nx, ny, nz = dg.shape
+    coef = workspace(dg.device, 14 * pdata.ncells, "cellfit")
+    _lib.call("ig_cell_fit_f64", _lib.ptr(rho), _lib.ptr(dg.xyz()), nx, ny, nz, _lib.ptr(pdata.cells),
+              pdata.ncells, _lib.ptr(coef), _lib.stream_handle(stream))
+    return coef
+
+
 def _corrected_delta(dg, pdata, rho, order, stream=None):
     """Quadratic sub-cell moments minus nodal-region moments (open_boundary.py:370-464)."""
     if pdata.ncells == 0:
         return torch.zeros(nmono(order), dtype=torch.float64, device=dg.device)
     lib = _lib.load()
     nx, ny, nz = dg.shape
-    coef = workspace(dg.device, 14 * pdata.ncells, "cellfit")
-    _lib.call("ig_cell_fit_f64", _lib.ptr(rho), _lib.ptr(dg.xyz()), nx, ny, nz, _lib.ptr(pdata.cells),
-              pdata.ncells, _lib.ptr(coef), _lib.stream_handle(stream))
+    coef = _cell_fit(dg, pdata, rho, stream)
     work = workspace(dg.device, lib.ig_physics_work_size(dg.N), "phys")
     quad = torch.zeros(nmono(order), dtype=torch.float64, device=dg.device)
     cptr, keep = _center_arg(dg.reference_center)
@@ -304,6 +311,48 @@ def _corrected_delta(dg, pdata, rho, order, stream=None):
     nodal = _moments_with_frac(dg, rho, order, pdata.frac, stream)
     del keep
     return quad - nodal
+
+
+def _split_correction(dg, pdata, rho, order, stream=None):
+    """Boundary-value correction of the `legacy_split` mode (open_boundary.py:559-719):
+    d = (rho_quadratic - rho_trilinear) J subvol at the sub-cell samples, expanded to the
+    same order about the |d|-weighted sample centre and evaluated on the faces.
+    Returns (field or None, centre or None)."""
+    if pdata.ncells == 0:
+        return None, None
+    lib = _lib.load()
+    nx, ny, nz = dg.shape
+    coef = _cell_fit(dg, pdata, rho, stream)
+    work = workspace(dg.device, lib.ig_physics_work_size(dg.N), "phys")
+    h = _lib.stream_handle(stream)
+    acc = torch.zeros(4, dtype=torch.float64, device=dg.device)
+    _lib.call("ig_cell_split_moments_f64", _lib.ptr(rho), _lib.ptr(dg.xyz()), _lib.ptr(dg.get("jac")), nx, ny, nz,
+              _lib.ptr(pdata.cells), pdata.ncells, _lib.ptr(coef), pdata.subvol, None, order, _lib.ptr(work),
+              _lib.ptr(acc), h)
+    a = acc.cpu().numpy()
+    if not a[0] > 0.0:
+        return None, None
+    center = a[1:] / a[0]
+    cptr, keep = _center_arg(center)
+    mom = torch.zeros(nmono(order), dtype=torch.float64, device=dg.device)
+    _lib.call("ig_cell_split_moments_f64", _lib.ptr(rho), _lib.ptr(dg.xyz()), _lib.ptr(dg.get("jac")), nx, ny, nz,
+              _lib.ptr(pdata.cells), pdata.ncells, _lib.ptr(coef), pdata.subvol, cptr, order, _lib.ptr(work),
+              _lib.ptr(mom), h)
+    del keep
+    return boundary_field(dg, mom.cpu().numpy(), order, center, stream), center
+
+
+def boundary_field(dg, flat, order, center=None, stream=None):
+    """Multipole potential of the flat raw moments `flat` (about `center`, default the grid
+    reference centre) on the six faces, zero inside (open_boundary.py:467-495)."""
+    nx, ny, nz = dg.shape
+    qcoef = torch.from_numpy(boundary_coefficients(np.asarray(flat, dtype=np.float64), order)).to(dg.device)
+    field = torch.empty(dg.N, dtype=torch.float64, device=dg.device)
+    cptr, keep = _center_arg(dg.reference_center if center is None else center)
+    _lib.call("ig_boundary_values_f64", _lib.ptr(dg.xyz()), nx, ny, nz, cptr, order, _lib.ptr(qcoef),
+              _lib.ptr(field), _lib.stream_handle(stream))
+    del keep
+    return field
 
 
 def _moments_with_frac(dg, rho, order, frac, stream=None):
@@ -331,19 +380,16 @@ def multipole_boundary_device(dg, rho, order=2, mode="corrected_moments", stream
     rho = rho.reshape(-1)
     base = _moments_with_frac(dg, rho, order, None, stream)
     moments = base
+    split = mode == "legacy_split" and dg.kind == "monitor"
     if dg.kind == "monitor":
-        if mode == "legacy_split":
-            raise NotImplementedError(
-                "monitor_boundary_construction_mode=`legacy_split` is not built on the HIP path; "
-                "use `corrected_moments` (the reference default).")
         moments = base + _corrected_delta(dg, poisson_data(dg), rho, order, stream)
-    qcoef = torch.from_numpy(boundary_coefficients(moments.cpu().numpy(), order)).to(dg.device)
-    nx, ny, nz = dg.shape
-    b = torch.empty(dg.N, dtype=torch.float64, device=dg.device)
-    cptr, keep = _center_arg(dg.reference_center)
-    _lib.call("ig_boundary_values_f64", _lib.ptr(dg.xyz()), nx, ny, nz, cptr, order, _lib.ptr(qcoef),
-              _lib.ptr(b), _lib.stream_handle(stream))
-    del keep
+    # legacy_split (open_boundary.py:923-934): faces from the nodal moments plus the split
+    # correction; the reported moments stay the corrected ones.
+    b = boundary_field(dg, (base if split else moments).cpu().numpy(), order, None, stream)
+    if split:
+        corr, _ = _split_correction(dg, poisson_data(dg), rho, order, stream)
+        if corr is not None:
+            b += corr
     return b, moments, base
 
 
@@ -605,12 +651,20 @@ def solve_open_boundary_poisson(grid_geometry, rho, multipole_order=2, tolerance
     if dg.kind == "monitor":
         bdip = base[1:4].copy() if order >= 1 else np.zeros(3)
         bquad = quadrupole_from_flat(base) if order >= 2 else np.zeros((3, 3))
+        # open_boundary.py:498-556: face values against the nodal-moment and the
+        # corrected-moment expansions.
+        faces = ~poisson_data(dg).interior.reshape(-1)
+        actual = sol["boundary"].reshape(-1)[faces]
+        corr = actual - boundary_field(dg, base, order).reshape(-1)[faces]
+        mis = actual - boundary_field(dg, flat, order).reshape(-1)[faces]
+        rms = lambda t: float(torch.sqrt(torch.mean(t * t)).item()) if t.numel() else 0.0  # noqa: E731
+        amax = lambda t: float(t.abs().max().item()) if t.numel() else 0.0  # noqa: E731
         diag = OpenBoundaryBoundaryConstructionDiagnostics(
             baseline_total_charge=float(base[0]), baseline_dipole_moment=bdip,
             baseline_quadrupole_tensor=bquad, correction_total_charge=float(flat[0] - base[0]),
             correction_dipole_moment=dip - bdip, correction_quadrupole_tensor=quad - bquad,
-            boundary_value_correction_rms=float("nan"), boundary_value_correction_max_abs=float("nan"),
-            corrected_moment_boundary_rms_mismatch=0.0, corrected_moment_boundary_max_abs_mismatch=0.0)
+            boundary_value_correction_rms=rms(corr), boundary_value_correction_max_abs=amax(corr),
+            corrected_moment_boundary_rms_mismatch=rms(mis), corrected_moment_boundary_max_abs_mismatch=amax(mis))
     bvals = sol["boundary"].cpu().numpy()
     bc = OpenBoundaryMultipoleBoundary(
         reference_center=tuple(float(c) for c in dg.reference_center), multipole_order=order,

@@ -139,20 +139,16 @@ def _design(x, y, z):
     return np.column_stack([np.ones_like(x), x, y, z, x * x, y * y, z * z, x * y, x * z, y * z])
 
 
-def corrected_moment_delta(geom, rho, center, order):
-    """Quadratic sub-cell moments minus nodal-region moments, open_boundary.py:370-464."""
-    mask, cells = selected_cells(geom)
-    zero = [np.zeros((3,) * l) if l else 0.0 for l in range(order + 1)]
-    if cells.size == 0:
-        return zero
-    frac = node_selected_fraction(mask)
-    w = rho * geom.cell_volumes * frac
-    nodal = raw_moments(w, geom.x_points - center[0], geom.y_points - center[1], geom.z_points - center[2], order)
+def _subcell_samples(geom, rho, cells):
+    """Per selected cell: least-squares quadratic over the clipped 4x4x4 stencil, sampled at
+    the 2x2x2 sub-cell midpoints (open_boundary.py:397-457, :584-626).  Returns the sample
+    positions, the trilinear Jacobian, the quadratic and the trilinear density values and
+    the sub-cell logical volume."""
     lx, ly, lz = geom.logical_x, geom.logical_y, geom.logical_z
     subvol = (lx[1] - lx[0]) * (ly[1] - ly[0]) * (lz[1] - lz[0]) / 8.0
     X, Y, Z, J = geom.x_points, geom.y_points, geom.z_points, geom.jacobian
     nx, ny, nz = geom.jacobian.shape
-    xs, ys, zs, ws = [], [], [], []
+    out = []
     for i, j, k in cells:
         sl = (slice(max(i - 1, 0), min(i + 3, nx)), slice(max(j - 1, 0), min(j + 3, ny)),
               slice(max(k - 1, 0), min(k + 3, nz)))
@@ -163,13 +159,41 @@ def corrected_moment_delta(geom, rho, center, order):
             xv, yv, zv = _trilinear(X[c], u, v, ww), _trilinear(Y[c], u, v, ww), _trilinear(Z[c], u, v, ww)
             jv = _trilinear(J[c], u, v, ww)
             rq = float(_design(np.array([xv]), np.array([yv]), np.array([zv]))[0] @ coef)
-            xs.append(xv)
-            ys.append(yv)
-            zs.append(zv)
-            ws.append(rq * jv * subvol)
-    xs, ys, zs, ws = map(np.asarray, (xs, ys, zs, ws))
-    quad = raw_moments(ws, xs - center[0], ys - center[1], zs - center[2], order)
+            out.append((xv, yv, zv, jv, rq, _trilinear(rho[c], u, v, ww)))
+    xs, ys, zs, js, rq, rt = (np.asarray(col, dtype=np.float64) for col in zip(*out))
+    return xs, ys, zs, js, rq, rt, subvol
+
+
+def corrected_moment_delta(geom, rho, center, order):
+    """Quadratic sub-cell moments minus nodal-region moments, open_boundary.py:370-464."""
+    mask, cells = selected_cells(geom)
+    zero = [np.zeros((3,) * l) if l else 0.0 for l in range(order + 1)]
+    if cells.size == 0:
+        return zero
+    frac = node_selected_fraction(mask)
+    w = rho * geom.cell_volumes * frac
+    nodal = raw_moments(w, geom.x_points - center[0], geom.y_points - center[1], geom.z_points - center[2], order)
+    xs, ys, zs, js, rq, _, subvol = _subcell_samples(geom, rho, cells)
+    quad = raw_moments(rq * js * subvol, xs - center[0], ys - center[1], zs - center[2], order)
     return [q - n for q, n in zip(quad, nodal)]
+
+
+def split_boundary_correction(geom, rho, order, bx, by, bz):
+    """Boundary-value correction of the `legacy_split` mode, open_boundary.py:559-719:
+    d = (rho_quadratic - rho_trilinear) J subvol at the sub-cell samples, expanded to the
+    same multipole order about the |d|-weighted centre of the samples.  Returns the
+    correction at the points (bx, by, bz) and the centre (None when there is none)."""
+    _, cells = selected_cells(geom)
+    if cells.size == 0:
+        return np.zeros_like(bx), None
+    xs, ys, zs, js, rq, rt, subvol = _subcell_samples(geom, rho, cells)
+    d = (rq - rt) * js * subvol
+    a = np.abs(d)
+    if a.sum() <= 0.0:
+        return np.zeros_like(bx), None
+    cc = np.array([np.sum(a * xs), np.sum(a * ys), np.sum(a * zs)]) / a.sum()
+    mom = raw_moments(d, xs - cc[0], ys - cc[1], zs - cc[2], order)
+    return boundary_potential(mom, bx - cc[0], by - cc[1], bz - cc[2], order), cc
 
 
 # ---------------------------------------------------------------- boundary condition
@@ -182,13 +206,26 @@ def multipole_boundary(geom, rho, order=2, mode="corrected_moments", center=None
     dx = geom.x_points - center[0]
     dy = geom.y_points - center[1]
     dz = geom.z_points - center[2]
-    moments = raw_moments(rho * geom.cell_volumes, dx, dy, dz, order)
-    if getattr(geom, "kind", "") == "monitor" and mode == "corrected_moments":
+    # "nodal" (nodal moments only, the construction the reference's recorded 67x67x81
+    # Hartree energy used) exists for pinning; the reference's modes are the other two.
+    if mode not in ("corrected_moments", "legacy_split", "nodal"):
+        raise ValueError(f"unknown monitor boundary construction mode {mode!r}")
+    baseline = raw_moments(rho * geom.cell_volumes, dx, dy, dz, order)
+    moments = baseline
+    monitor = getattr(geom, "kind", "") == "monitor" and mode != "nodal"
+    if monitor:
         delta = corrected_moment_delta(geom, rho, center, order)
-        moments = [m + d for m, d in zip(moments, delta)]
+        moments = [m + d for m, d in zip(baseline, delta)]
     bmask = _boundary_mask(geom.cell_volumes.shape)
     b = np.zeros(geom.cell_volumes.shape)
-    b[bmask] = boundary_potential(moments, dx[bmask], dy[bmask], dz[bmask], order)
+    if monitor and mode == "legacy_split":
+        # open_boundary.py:923-934: nodal-moment boundary plus the split correction; the
+        # reported moments stay the corrected ones.
+        corr, _ = split_boundary_correction(geom, rho, order, geom.x_points[bmask], geom.y_points[bmask],
+                                            geom.z_points[bmask])
+        b[bmask] = boundary_potential(baseline, dx[bmask], dy[bmask], dz[bmask], order) + corr
+    else:
+        b[bmask] = boundary_potential(moments, dx[bmask], dy[bmask], dz[bmask], order)
     return b, moments
 
 

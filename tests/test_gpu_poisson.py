@@ -53,6 +53,67 @@ def test_boundary_values_match_oracle(small_monitor, order):
         assert np.max(np.abs(flat - ref_flat)) <= 1e-12 * np.max(np.abs(ref_flat)), (name, order)
 
 
+@pytest.mark.parametrize("tag", ["centered", "shifted"])
+@pytest.mark.parametrize("mode", ["corrected_moments", "legacy_split"])
+def test_monitor_boundary_modes_match_reference_vectors(golden, small_monitor, tag, mode):
+    """Both monitor boundary constructions against the reference-generated vectors
+    (open_boundary.py:794-987 on the 19x19x23 H2 monitor grid); the two modes differ by
+    up to 3e-5 there, far above the 1e-12 bar."""
+    rho = golden["rho_" + tag]
+    dg = DeviceGrid(small_monitor)
+    b, moms, _ = poisson.multipole_boundary_device(dg, torch.from_numpy(rho).cuda(), 2, mode)
+    key = f"{tag}_{mode}"
+    np.testing.assert_allclose(b.cpu().numpy().reshape(rho.shape), golden[key + "_boundary"], rtol=1e-12,
+                               atol=1e-14)
+    flat = moms.cpu().numpy()
+    assert abs(flat[0] - float(golden[key + "_charge"])) < 1e-13
+    np.testing.assert_allclose(flat[1:4], golden[key + "_dipole"], atol=1e-13)
+    np.testing.assert_allclose(poisson.quadrupole_from_flat(flat), golden[key + "_quadrupole"], atol=1e-12)
+
+
+@pytest.mark.parametrize("order", [0, 1, 2, 3, 4])
+def test_legacy_split_boundary_matches_oracle(small_monitor, order):
+    for name, geom in list(_cases(small_monitor))[:2]:
+        rho = _density(geom)
+        dg = DeviceGrid(geom)
+        b, _, _ = poisson.multipole_boundary_device(dg, torch.from_numpy(rho).cuda(), order, "legacy_split")
+        ref_b, _ = opoisson.multipole_boundary(geom, rho, order, "legacy_split")
+        err = np.max(np.abs(b.cpu().numpy().reshape(rho.shape) - ref_b)) / np.max(np.abs(ref_b))
+        print(name, order, "legacy_split boundary rel err", err)
+        assert err <= 1e-12, (name, order, err)
+
+
+@pytest.mark.parametrize("mode", ["corrected_moments", "legacy_split"])
+def test_boundary_construction_diagnostics(golden, small_monitor, mode):
+    """open_boundary.py:498-556: RMS / max of (actual - nodal-moment faces) and of
+    (actual - corrected-moment faces), from the reference vectors and the oracle's
+    nodal-moment faces."""
+    geom = small_monitor
+    rho = golden["rho_shifted"]
+    res = poisson.solve_open_boundary_poisson(geom, rho, monitor_boundary_construction_mode=mode)
+    d = res.boundary_condition.diagnostics
+    c = geom.reference_center
+    dx, dy, dz = geom.x_points - c[0], geom.y_points - c[1], geom.z_points - c[2]
+    faces = np.ones(rho.shape, dtype=bool)
+    faces[1:-1, 1:-1, 1:-1] = False
+    base = opoisson.raw_moments(rho * geom.cell_volumes, dx, dy, dz, 2)
+    nodal_faces = opoisson.boundary_potential(base, dx[faces], dy[faces], dz[faces], 2)
+    actual = golden[f"shifted_{mode}_boundary"][faces]
+    corr = actual - nodal_faces
+    mis = actual - golden["shifted_corrected_moments_boundary"][faces]
+    rms = lambda v: float(np.sqrt(np.mean(v * v)))  # noqa: E731
+    # face values carry ~1e-13 absolute differences from the reference vectors
+    assert abs(d.boundary_value_correction_rms - rms(corr)) <= 1e-11
+    assert abs(d.boundary_value_correction_max_abs - np.max(np.abs(corr))) <= 1e-11
+    assert abs(d.corrected_moment_boundary_rms_mismatch - rms(mis)) <= 1e-11
+    assert abs(d.corrected_moment_boundary_max_abs_mismatch - np.max(np.abs(mis))) <= 1e-11
+    assert abs(d.baseline_total_charge - base[0]) <= 1e-13
+    if mode == "corrected_moments":
+        assert d.corrected_moment_boundary_max_abs_mismatch <= 1e-14
+    else:
+        assert d.corrected_moment_boundary_max_abs_mismatch > 1e-6
+
+
 def test_fp64_solution_matches_direct_solve(small_monitor):
     for name, geom in _cases(small_monitor):
         rho = _density(geom)

@@ -90,11 +90,12 @@ def test_monitor_generator_matches_reference(golden):
     np.testing.assert_allclose(g.patch_radii, golden["mon_patch_radii"], rtol=1e-15)
 
 
+@pytest.mark.parametrize("mode", ["corrected_moments", "legacy_split"])
 @pytest.mark.parametrize("tag", ["centered", "shifted"])
-def test_corrected_moment_boundary_matches_reference(golden, small_monitor, tag):
+def test_monitor_boundary_matches_reference(golden, small_monitor, tag, mode):
     rho = golden["rho_" + tag]
-    b, mom = poisson.multipole_boundary(small_monitor, rho, 2, "corrected_moments")
-    key = f"{tag}_corrected_moments"
+    b, mom = poisson.multipole_boundary(small_monitor, rho, 2, mode)
+    key = f"{tag}_{mode}"
     assert abs(mom[0] - float(golden[key + "_charge"])) < 1e-13
     np.testing.assert_allclose(mom[1], golden[key + "_dipole"], atol=1e-13)
     np.testing.assert_allclose(poisson.traceless_quadrupole(mom[2]), golden[key + "_quadrupole"], atol=1e-12)
