@@ -104,7 +104,7 @@ int ig_bicgstab_x_f32(long long n, float* x, const float* phat, const float* sha
 /* Doubles of `work` required by ig_gram_* for an (m x n) product over N points. */
 long long ig_gram_work_size(int m, int n, long long N);
 
-/* G[m x n] (=|+=) X W Y^T, X (m, N) and Y (n, N) orbital-major, w[N]; FP32 inputs on
+/* G[m x n] (=|+=) X W Y^T, X (m, N) and Y (n, N) orbital-major, w[N] (NULL: W = I); FP32 inputs on
  * MFMA (v_mfma_f32_32x32x2f32) with FP64 split-K reduction; FP64 inputs on FP64 FMAs.
  * Replaces: ks/eigensolver.py:246-272 (weighted_overlap_matrix),
  * ops/reductions_jax.py:89-104, :150-160, ks/eigensolver_jax_cache.py:95-113 and the
@@ -229,6 +229,36 @@ int ig_cell_split_moments_f64(const double* rho, const double* xyz, const double
                               int nz, const int* cells, int ncells, const double* coef, double subvol,
                               const double* center, int order, double* work, double* out,
                               hipStream_t stream);
+
+/* ---- near-core local-GTH patch (pseudo/local.py:379-703) ------------------------- */
+
+/* Bytes of device workspace ig_knn_f64 needs for nq queries and k neighbours (k <= 32),
+ * or -1 for invalid sizes. */
+long long ig_knn_work_bytes(int nq, int k);
+/* k nearest grid nodes of each query point (queries = [3, nq] device SoA, xyz = [3, N]):
+ * idx[nq, k] (int32) and Euclidean dist[nq, k], ascending by (distance, node index).
+ * Replaces: pseudo/local.py:416-440 (cKDTree(point_cloud).query(patch_points, k)); the
+ * neighbour sets depend on the geometry only and are computed once per patch. */
+int ig_knn_f64(const double* xyz, long long N, const double* queries, int nq, int k, void* work,
+               long long work_bytes, int* idx, double* dist, hipStream_t stream);
+/* values[t] = inverse-distance-squared interpolation of field at query t (exact hit when
+ * dist[t, 0] <= 1e-12, weights 1 / max(d, 1e-12)^2 normalised), values may be NULL;
+ * out[0] = sum_t values[t] vq[t].  work: ig_physics_work_size doubles.
+ * Replaces: pseudo/local.py:442-458 and the patch quadrature of :489-492. */
+int ig_idw_quadrature_f64(const double* field, const int* idx, const double* dist, int nq, int k,
+                          const double* vq, double* values, double* work, double* out,
+                          hipStream_t stream);
+/* Main-grid quadrature over the sphere |r - R_A| <= radius of one atom (atom: host pointer
+ * to 9 doubles x, y, z, Z_ion, r_loc, C1..C4): out[3] = {sum rho V_A w, sum rho w, node count}.
+ * Replaces: pseudo/local.py:494-510 and the density weight of :572-581. */
+int ig_patch_main_f64(const double* xyz, const double* rho, const double* w, long long N,
+                      const double* atom, double radius, double* work, double* out,
+                      hipStream_t stream);
+/* out += scale * V_A (constant == 0) or out += scale (constant != 0) on the nodes with
+ * |r - R_A| <= radius (atom: host pointer as above).  Replaces: pseudo/local.py:546-582
+ * (_build_atomic_frozen_patch_correction_field). */
+int ig_patch_field_f64(const double* xyz, long long N, const double* atom, double radius,
+                       int constant, double scale, double* out, hipStream_t stream);
 
 #ifdef __cplusplus
 }

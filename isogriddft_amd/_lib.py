@@ -56,8 +56,13 @@ SIGNATURES = {
     "ig_cell_fit_f64": [P, P, I, I, I, P, I, P, P],
     "ig_cell_moments_f64": [P, P, I, I, I, P, I, P, D, P, I, P, P, P],
     "ig_cell_split_moments_f64": [P, P, P, I, I, I, P, I, P, D, P, I, P, P, P],
+    "ig_knn_work_bytes": [I, I],
+    "ig_knn_f64": [P, L, P, I, I, P, L, P, P, P],
+    "ig_idw_quadrature_f64": [P, P, P, I, I, P, P, P, P, P],
+    "ig_patch_main_f64": [P, P, P, L, P, D, P, P, P],
+    "ig_patch_field_f64": [P, L, P, D, I, D, P, P],
 }
-_RESTYPE = {"ig_gram_work_size": L}
+_RESTYPE = {"ig_gram_work_size": L, "ig_knn_work_bytes": L}
 
 _lib = None
 

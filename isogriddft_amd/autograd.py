@@ -10,8 +10,10 @@ on the same HIP kernels as the forward path:
                       d/dV = sum_m psi_m g_m.
 * ``RitzValues``      lowest-k eigenvalues of the fixed-potential operator T + V (block
                       iteration); backward is first-order perturbation theory
-                      (Hellmann-Feynman): d eps_i / d V_p = w_p psi_i(p)^2 with
-                      W-normalised Ritz vectors, i.e. dE/dv_ext for E = sum_i g_i eps_i.
+                      (Hellmann-Feynman): d eps_i / d V_p = w_p psi_i(p)^2 with Ritz
+                      vectors normalised under the weights w that make T + V self-adjoint
+                      (cell volumes on monitor grids, (h s_x)(h s_y)(h s_z) on legacy
+                      grids), i.e. dE/dv_ext for E = sum_i g_i eps_i.
 * ``IonicEnergy``     E_ion(R) = E_loc + E_nl + E_II at frozen density and orbitals as a
                       function of the atomic positions; backward gives dE/dR (minus the
                       Hellmann-Feynman forces): ig_vloc_gth_grad_f64 for the local part,
@@ -69,7 +71,7 @@ class RitzValues(torch.autograd.Function):
         from .hamiltonian import BlockHamiltonian
 
         H = BlockHamiltonian(dg, veff.detach().reshape(-1), closure=closure)
-        res = block_subspace_iteration(H, dg.weights(), X0, k, tol, max_iter)
+        res = block_subspace_iteration(H, dg.operator_weights(), X0, k, tol, max_iter)
         if not res.converged:
             raise RuntimeError("the block eigensolver did not converge inside the autograd forward")
         psi = res.orbitals.reshape(k, -1).to(torch.float64)
@@ -81,7 +83,8 @@ class RitzValues(torch.autograd.Function):
     @staticmethod
     def backward(ctx, g):
         (psi,) = ctx.saved_tensors
-        w = ctx.dg.weights()
+        # psi is normalised in the inner product that makes T + V self-adjoint
+        w = ctx.dg.operator_weights()
         grad = w * (g.to(psi.device)[:, None] * psi * psi).sum(0)
         return grad.reshape(ctx.vshape), None, None, None, None, None, None
 

@@ -1,7 +1,7 @@
 // Weighted tall-skinny Gram products G = X W Y^T for orbital-major blocks.
 //
 // X is (m, N) and Y is (n, N) with each orbital contiguous (row stride ldx / ldy),
-// W = diag(w) the cell-volume metric.  This is the projected-matrix / overlap
+// W = diag(w) the cell-volume metric (w == NULL: W = I).  This is the projected-matrix / overlap
 // product of the reference eigensolver (ks/eigensolver.py:246-272,
 // ops/reductions_jax.py:89-98, ks/eigensolver_jax.py:104-105).
 //
@@ -59,7 +59,7 @@ __global__ __launch_bounds__(IG_THREADS) void gram_f32_mfma(const float* __restr
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
   for (long long k = kbeg; k < kend; k += KT) {
-    stage_tile<float, KT>(sx, X, ldx, m, i0, k, kend, w, true);
+    stage_tile<float, KT>(sx, X, ldx, m, i0, k, kend, w, w != nullptr);
     stage_tile<float, KT>(sy, Y, ldy, n, j0, k, kend, w, false);
     __syncthreads();
     const float* ax = sx + (wm * 32 + (lane & 31)) * (KT + 1) + (lane >> 5);
@@ -102,7 +102,7 @@ __global__ __launch_bounds__(IG_THREADS) void gram_f64_simt(const double* __rest
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[a][c] = 0.0;
   for (long long k = kbeg; k < kend; k += KT) {
-    stage_tile<double, KT>(sx, X, ldx, m, i0, k, kend, w, true);
+    stage_tile<double, KT>(sx, X, ldx, m, i0, k, kend, w, w != nullptr);
     stage_tile<double, KT>(sy, Y, ldy, n, j0, k, kend, w, false);
     __syncthreads();
 #pragma unroll 4
@@ -176,7 +176,7 @@ long long ig_gram_work_size(int m, int n, long long N) {
 int ig_gram_f32(const float* X, long long ldx, int m, const float* Y, long long ldy, int n,
                 const float* w, long long N, double* G, int accumulate, double* work,
                 long long work_elems, hipStream_t s) {
-  if (!X || !Y || !w || !G || !work || m <= 0 || n <= 0 || N <= 0) return IG_EINVAL;
+  if (!X || !Y || !G || !work || m <= 0 || n <= 0 || N <= 0) return IG_EINVAL;
   if (ldx < N || ldy < N) return IG_EINVAL;
   const GramPlan g = plan(m, n, N);
   if (work_elems < (long long)g.nkb * m * n) return IG_EINVAL;
@@ -192,7 +192,7 @@ int ig_gram_f32(const float* X, long long ldx, int m, const float* Y, long long 
 int ig_gram_f64(const double* X, long long ldx, int m, const double* Y, long long ldy, int n,
                 const double* w, long long N, double* G, int accumulate, double* work,
                 long long work_elems, hipStream_t s) {
-  if (!X || !Y || !w || !G || !work || m <= 0 || n <= 0 || N <= 0) return IG_EINVAL;
+  if (!X || !Y || !G || !work || m <= 0 || n <= 0 || N <= 0) return IG_EINVAL;
   if (ldx < N || ldy < N) return IG_EINVAL;
   const GramPlan g = plan(m, n, N);
   if (work_elems < (long long)g.nkb * m * n) return IG_EINVAL;

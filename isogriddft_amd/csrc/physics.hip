@@ -8,6 +8,7 @@
 //    and the corrected-moment quadratic sub-cell quadrature
 //                                                        (poisson/open_boundary.py:223-495)
 #include "common.h"
+#include "gth.h"
 
 namespace {
 
@@ -46,30 +47,6 @@ inline int red_blocks(long long n) {
 }
 
 // ------------------------------------------------------------------ local GTH potential
-
-struct GthAtom {
-  double x, y, z, zion, rloc, c[4];
-};
-
-__device__ __forceinline__ double gth_vloc(double r, const GthAtom& a) {
-  const double s = r / (1.4142135623730951 * a.rloc);
-  const double coul = (r > 0.0) ? -a.zion * erf(s) / r : -a.zion * sqrt(2.0 / PI) / a.rloc;
-  const double x2 = (r / a.rloc) * (r / a.rloc);
-  const double poly = a.c[0] + x2 * (a.c[1] + x2 * (a.c[2] + x2 * a.c[3]));
-  return coul + exp(-0.5 * x2) * poly;
-}
-
-// d V_loc / d r
-__device__ __forceinline__ double gth_dvloc(double r, const GthAtom& a) {
-  const double al = 1.0 / (1.4142135623730951 * a.rloc);
-  double dcoul = 0.0;
-  if (r > 0.0) dcoul = a.zion * (erf(al * r) / (r * r) - 2.0 * al / sqrt(PI) * exp(-(al * r) * (al * r)) / r);
-  const double x = r / a.rloc;
-  const double x2 = x * x;
-  const double poly = a.c[0] + x2 * (a.c[1] + x2 * (a.c[2] + x2 * a.c[3]));
-  const double dpoly = x * (2.0 * a.c[1] + x2 * (4.0 * a.c[2] + x2 * 6.0 * a.c[3]));
-  return dcoul + exp(-0.5 * x2) * (dpoly - x * poly) / a.rloc;
-}
 
 __global__ __launch_bounds__(IG_THREADS) void vloc_kernel(const double* __restrict__ xyz, long long N, int natoms,
                                                           const GthAtom* __restrict__ atoms, double* __restrict__ out,

@@ -94,6 +94,12 @@ class DeviceGrid:
                 parts += [am, ap]
             self._t[("lcoef", f64)] = torch.from_numpy(np.concatenate(parts)).to(dev)
             self.h = tuple(_grid.uniform_spacing(a) for a in _grid.logical_axes(geom))
+            # The 7-point flux operator is self-adjoint under the product weights
+            # (h s_x)(h s_y)(h s_z), not under the midpoint cell volumes used for integrals.
+            sx, sy, sz = (h * np.asarray(s, dtype=np.float64) for h, s in
+                          zip(self.h, (geom.x_point_jacobian, geom.y_point_jacobian, geom.z_point_jacobian)))
+            wsa = sx[:, None, None] * sy[None, :, None] * sz[None, None, :]
+            self._t[("wsa", f64)] = torch.from_numpy(np.ascontiguousarray(wsa.reshape(-1))).to(dev)
             self.box_bounds = None
         self.patch_radii = tuple(getattr(geom, "patch_radii", ()) or ())
 
@@ -113,6 +119,50 @@ class DeviceGrid:
 
     def weights(self, dtype=torch.float64):
         return self.get("w", dtype)
+
+    def kinetic_diagonal(self):
+        """Diagonal of the discrete Laplacian (FP64, flat), cached.  Legacy grids: exact,
+        -(a-_a + a+_a) summed over axes.  Monitor grids: the centred-difference form
+        J^-1 sum_a d_a(c^aa d_a) gives -J^-1 sum_a (c^aa_{i+e_a} + c^aa_{i-e_a}) / (4 h_a^2)
+        from the in-domain neighbours (the mixed terms have no diagonal entry); the
+        one-sided boundary rows of the production closure are not represented."""
+        key = ("ldiag", torch.float64)
+        if key in self._t:
+            return self._t[key]
+        shape = self.shape
+        if self.kind == "monitor":
+            sc = self.get("scoef").view(6, *shape)
+            d = torch.zeros(shape, dtype=torch.float64, device=self.device)
+            for a in range(3):
+                c = sc[a]
+                n = shape[a]
+                lo = [slice(None)] * 3
+                hi = [slice(None)] * 3
+                lo[a] = slice(0, n - 1)
+                hi[a] = slice(1, n)
+                d[tuple(lo)] += c[tuple(hi)]   # c_{i+1} for i < n-1
+                d[tuple(hi)] += c[tuple(lo)]   # c_{i-1} for i > 0
+            diag = -(d * self.get("invj").view(shape)).reshape(-1)
+        else:
+            lc = self.get("lcoef")
+            nx, ny, nz = shape
+            off = 0
+            diag = torch.zeros(shape, dtype=torch.float64, device=self.device)
+            for a, n in enumerate((nx, ny, nz)):
+                am, ap = lc[off:off + n], lc[off + n:off + 2 * n]
+                off += 2 * n
+                v = -(am + ap)
+                view = [1, 1, 1]
+                view[a] = n
+                diag += v.view(view)
+            diag = diag.reshape(-1)
+        self._t[key] = diag.contiguous()
+        return self._t[key]
+
+    def operator_weights(self, dtype=torch.float64):
+        """Weights under which the trial_fix / legacy kinetic operator is self-adjoint:
+        the cell volumes on monitor grids, (h s_x)(h s_y)(h s_z) on legacy grids."""
+        return self.get("w" if self.kind == "monitor" else "wsa", dtype)
 
     def xyz(self):
         return self.get("xyz")

@@ -68,6 +68,11 @@ class BlockHamiltonian:
 
     __call__ = apply
 
+    def diagonal(self) -> torch.Tensor:
+        """Diagonal of -1/2 Lap + V_eff (FP64, flat; the nonlocal term is left out): the
+        Davidson preconditioner of the block eigensolver."""
+        return -0.5 * self.dg.kinetic_diagonal() + self.veff[torch.float64]
+
 
 # ------------------------------------------------------------------ NumPy call surfaces
 

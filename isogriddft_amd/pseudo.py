@@ -9,8 +9,10 @@ Nonlocal part: separable projectors p_i^l(r) Y_lm on a per-atom support box
 grid; here each projector lives on the index box of the nodes within 10 r_l of the
 atom (outside it |p| < e^-50 of its peak), is evaluated once per (grid, atoms) and
 kept in HBM, and the block apply is project -> h-matrix -> scatter on that box.
-The near-core patch correction field (pseudo/local.py:379-703) is not built here;
-callers pass it (or any extra local field) as an explicit potential.
+Near-core patch (pseudo/local.py:379-703): per atom a fine Cartesian lattice inside the
+patch sphere samples the density by inverse-distance-squared interpolation from its k
+nearest main-grid nodes (found once per grid on the GPU, ig_knn_f64); the patch
+quadrature minus the main-grid quadrature is embedded as a frozen correction field.
 """
 
 from __future__ import annotations
@@ -170,6 +172,304 @@ def evaluate_local_ionic_energy(evaluation: LocalIonicPotentialEvaluation, densi
     a = torch.from_numpy(np.ascontiguousarray(rho)).to(dg.device)
     b = torch.from_numpy(np.ascontiguousarray(evaluation.total_local_potential)).to(dg.device)
     return wdot(dg, a, b)
+
+
+# ------------------------------------------------------------------ near-core patch
+# pseudo/local.py:379-703.  The patch lattice, its sphere mask and the k nearest main-grid
+# nodes of every patch point depend on the geometry only: they are built once per
+# (grid, atoms, patch parameters) and kept in HBM; each density is then one gather +
+# reduction per atom (ig_idw_quadrature_f64), one masked main-grid reduction
+# (ig_patch_main_f64) and, for the frozen embedding, one masked field update.
+
+
+@dataclass(frozen=True)
+class LocalPotentialPatchParameters:
+    """pseudo/local.py:55-76 (defaults and validation)."""
+
+    patch_radius_scale: float = 1.0
+    patch_grid_shape: tuple = (25, 25, 25)
+    correction_strength: float = 1.0
+    interpolation_neighbors: int = 8
+
+    def __post_init__(self) -> None:
+        if self.patch_radius_scale <= 0.0:
+            raise ValueError("patch_radius_scale must be positive.")
+        if self.correction_strength < 0.0:
+            raise ValueError("correction_strength must be non-negative.")
+        if self.interpolation_neighbors <= 0:
+            raise ValueError("interpolation_neighbors must be positive.")
+        if len(self.patch_grid_shape) != 3:
+            raise ValueError("patch_grid_shape must be a 3-tuple.")
+        for axis_points in self.patch_grid_shape:
+            if axis_points < 3 or axis_points % 2 == 0:
+                raise ValueError("Each patch-grid direction must use an odd number of points >= 3.")
+
+
+@dataclass(frozen=True)
+class AtomicLocalPotentialPatchCorrection:
+    """pseudo/local.py:79-92."""
+
+    atom_index: int
+    element: str
+    position: tuple
+    patch_radius_bohr: float
+    patch_grid_shape: tuple
+    correction_strength: float
+    main_grid_patch_energy: float
+    patch_integrated_energy: float
+    correction_energy: float
+    patch_point_count: int
+
+
+@dataclass(frozen=True)
+class LocalIonicPotentialPatchEvaluation:
+    """pseudo/local.py:95-113."""
+
+    base_evaluation: LocalIonicPotentialEvaluation
+    density_field: np.ndarray
+    patch_parameters: LocalPotentialPatchParameters
+    atomic_patch_corrections: tuple
+    uncorrected_local_energy: float
+    corrected_local_energy: float
+    total_patch_correction: float
+
+
+@dataclass(frozen=True)
+class FrozenPatchLocalPotentialEmbedding:
+    """pseudo/local.py:116-135."""
+
+    base_evaluation: LocalIonicPotentialEvaluation
+    patch_evaluation: LocalIonicPotentialPatchEvaluation
+    corrected_total_local_potential: np.ndarray
+    correction_field: np.ndarray
+    atomic_scaling_factors: tuple
+    embedded_patch_correction_energy: float
+    embedding_energy_mismatch: float
+
+
+def geometry_patch_radii(grid_geometry) -> tuple:
+    """Per-atom patch radii of a monitor grid: the reference's `patch_interfaces`
+    (which must implement `local_gth`, pseudo/local.py:467-471) or `patch_radii`."""
+    interfaces = getattr(grid_geometry, "patch_interfaces", None)
+    if interfaces is not None:
+        radii = []
+        for itf in interfaces:
+            if "local_gth" not in tuple(getattr(itf, "implemented_purposes", ("local_gth",))):
+                raise ValueError("This monitor-grid patch interface does not implement local-GTH correction.")
+            radii.append(float(itf.patch_radius))
+        return tuple(radii)
+    radii = tuple(float(r) for r in (getattr(grid_geometry, "patch_radii", ()) or ()))
+    if not radii:
+        raise ValueError("The near-core patch correction needs a monitor grid with per-atom patch radii.")
+    return radii
+
+
+def _host_atom(row):
+    arr = (ctypes.c_double * 9)(*[float(v) for v in row])
+    return ctypes.cast(arr, ctypes.c_void_p), arr
+
+
+class _AtomPatch:
+    """Geometry-only data of one atom's patch: lattice points inside the sphere (device
+    SoA), V_A there, and the k nearest main-grid nodes of each point."""
+
+    def __init__(self, dg: DeviceGrid, atom_index, element, position, radius, shape, neighbors):
+        self.atom_index = atom_index
+        self.element = element
+        self.position = tuple(float(c) for c in position)
+        self.radius = float(radius)
+        self.shape = tuple(int(n) for n in shape)
+        self.k = int(neighbors)
+        R = self.radius
+        # pseudo/local.py:379-413 (np.linspace lattice, cell volume from the first steps)
+        axes = [np.linspace(c - R, c + R, n, dtype=np.float64) for c, n in zip(self.position, self.shape)]
+        self.cell_volume = float(axes[0][1] - axes[0][0]) * float(axes[1][1] - axes[1][0]) * \
+            float(axes[2][1] - axes[2][0])
+        PX, PY, PZ = np.meshgrid(*axes, indexing="ij")
+        dx, dy, dz = PX - self.position[0], PY - self.position[1], PZ - self.position[2]
+        mask = np.sqrt(dx * dx + dy * dy + dz * dz, dtype=np.float64) <= R
+        pts = np.ascontiguousarray(np.stack([PX[mask], PY[mask], PZ[mask]]))
+        self.nq = int(pts.shape[1])
+        dev = dg.device
+        self.row = _atom_rows([(element, self.position)])[0]
+        self.q = torch.from_numpy(pts).to(dev)
+        self.vq = torch.empty(self.nq, dtype=torch.float64, device=dev)
+        self.idx = torch.empty((self.nq, self.k), dtype=torch.int32, device=dev)
+        self.dist = torch.empty((self.nq, self.k), dtype=torch.float64, device=dev)
+        if self.nq == 0:
+            return
+        rows = torch.from_numpy(self.row[None, :].copy()).to(dev)
+        _lib.call("ig_vloc_gth_f64", _lib.ptr(self.q), self.nq, 1, _lib.ptr(rows), _lib.ptr(self.vq), 0,
+                  _lib.stream_handle())
+        if self.k > dg.N:
+            raise ValueError("interpolation_neighbors exceeds the number of grid nodes.")
+        lib = _lib.load()
+        nbytes = int(lib.ig_knn_work_bytes(self.nq, self.k))
+        if nbytes <= 0:
+            raise ValueError("interpolation_neighbors must be between 1 and 32 on the HIP path.")
+        work = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        _lib.call("ig_knn_f64", _lib.ptr(dg.xyz()), dg.N, _lib.ptr(self.q), self.nq, self.k, _lib.ptr(work),
+                  nbytes, _lib.ptr(self.idx), _lib.ptr(self.dist), _lib.stream_handle())
+        del work
+
+    def quadrature(self, dg, rho, values=None, stream=None):
+        """(sum over patch points of rho_interp V_A) * patch cell volume, on the device."""
+        out = torch.zeros(1, dtype=torch.float64, device=dg.device)
+        if self.nq == 0:
+            return out
+        lib = _lib.load()
+        work = workspace(dg.device, lib.ig_physics_work_size(dg.N), "phys")
+        _lib.call("ig_idw_quadrature_f64", _lib.ptr(rho), _lib.ptr(self.idx), _lib.ptr(self.dist), self.nq, self.k,
+                  _lib.ptr(self.vq), _lib.ptr(values), _lib.ptr(work), _lib.ptr(out), _lib.stream_handle(stream))
+        return out * self.cell_volume
+
+    def main_grid(self, dg, rho, stream=None):
+        """[sum rho V_A w, sum rho w, node count] over the main-grid nodes inside the sphere."""
+        lib = _lib.load()
+        work = workspace(dg.device, lib.ig_physics_work_size(dg.N), "phys")
+        out = torch.zeros(3, dtype=torch.float64, device=dg.device)
+        aptr, keep = _host_atom(self.row)
+        _lib.call("ig_patch_main_f64", _lib.ptr(dg.xyz()), _lib.ptr(rho), _lib.ptr(dg.weights()), dg.N, aptr,
+                  self.radius, _lib.ptr(work), _lib.ptr(out), _lib.stream_handle(stream))
+        del keep
+        return out
+
+    def add_field(self, dg, out, scale, constant, stream=None):
+        aptr, keep = _host_atom(self.row)
+        _lib.call("ig_patch_field_f64", _lib.ptr(dg.xyz()), dg.N, aptr, self.radius, int(constant), float(scale),
+                  _lib.ptr(out), _lib.stream_handle(stream))
+        del keep
+
+
+class PatchPlan:
+    """All atoms' patches for one grid and parameter set."""
+
+    def __init__(self, dg: DeviceGrid, atoms, radii, params: LocalPotentialPatchParameters):
+        if len(radii) != len(atoms):
+            raise ValueError("The grid's patch interfaces do not match the atoms of the case.")
+        self.params = params
+        self.patches = [
+            _AtomPatch(dg, n, el, pos, radii[n] * params.patch_radius_scale, params.patch_grid_shape,
+                       params.interpolation_neighbors)
+            for n, (el, pos) in enumerate(atoms)
+        ]
+
+    def corrections(self, dg, rho, stream=None):
+        """Per atom (pseudo/local.py:461-517): main-grid and patch energies, correction
+        energy, density weight and node count inside the sphere (host floats)."""
+        res = []
+        for p in self.patches:
+            pe = p.quadrature(dg, rho, stream=stream)
+            mg = p.main_grid(dg, rho, stream)
+            vals = torch.cat([pe, mg]).cpu().numpy()
+            patch_e, main_e, dw, cnt = (float(v) for v in vals)
+            ce = self.params.correction_strength * (patch_e - main_e)
+            res.append((p, main_e, patch_e, ce, dw, int(round(cnt))))
+        return res
+
+    def embed(self, dg, corrections, out=None, stream=None):
+        """Frozen correction field (pseudo/local.py:546-582) accumulated into `out`;
+        returns (field, per-atom scaling factors)."""
+        if out is None:
+            out = torch.zeros(dg.N, dtype=torch.float64, device=dg.device)
+        scalings = []
+        for p, main_e, _, ce, dw, cnt in corrections:
+            if cnt == 0 or abs(ce) <= 1.0e-16:
+                scalings.append(0.0)
+                continue
+            if abs(main_e) > 1.0e-14:
+                scale = ce / main_e
+                p.add_field(dg, out, scale, False, stream)
+            elif dw > 1.0e-16:
+                scale = ce / dw
+                p.add_field(dg, out, scale, True, stream)
+            else:
+                scale = 0.0
+            scalings.append(float(scale))
+        return out, tuple(scalings)
+
+
+_PATCH_PLANS: dict = {}
+
+
+def patch_plan(dg: DeviceGrid, case, grid_geometry, params: LocalPotentialPatchParameters) -> PatchPlan:
+    atoms = case_atoms(case)
+    radii = geometry_patch_radii(grid_geometry)
+    key = (id(dg), tuple(atoms), radii, params)
+    plan = _PATCH_PLANS.get(key)
+    if plan is None:
+        plan = PatchPlan(dg, atoms, radii, params)
+        _PATCH_PLANS[key] = plan
+    return plan
+
+
+def frozen_patch_field_device(dg: DeviceGrid, case, grid_geometry, rho, params=None, stream=None):
+    """Device-level frozen patch embedding for a flat FP64 density: (correction field,
+    per-atom correction records, scaling factors)."""
+    params = LocalPotentialPatchParameters() if params is None else params
+    plan = patch_plan(dg, case, grid_geometry, params)
+    corr = plan.corrections(dg, rho.reshape(-1), stream)
+    field, scalings = plan.embed(dg, corr, stream=stream)
+    return field, corr, scalings
+
+
+def _patch_density(density_field, dg):
+    rho = np.asarray(density_field)
+    if rho.ndim != 3:
+        raise ValueError(f"density_field must be a 3D field; received ndim={rho.ndim}.")
+    if rho.shape != dg.shape:
+        raise ValueError(f"density_field must match the grid shape {dg.shape}; received {rho.shape}.")
+    return np.asarray(rho, dtype=np.float64)
+
+
+def evaluate_monitor_grid_local_ionic_potential_with_patch(case, grid_geometry, density_field, *,
+                                                           patch_parameters=None, base_evaluation=None):
+    """pseudo/local.py:520-582 on the GPU."""
+    dg = device_grid(grid_geometry)
+    density = _patch_density(density_field, dg)
+    params = LocalPotentialPatchParameters() if patch_parameters is None else patch_parameters
+    if base_evaluation is None:
+        base_evaluation = evaluate_local_ionic_potential(case, grid_geometry)
+    uncorrected = evaluate_local_ionic_energy(base_evaluation, density)
+    plan = patch_plan(dg, case, grid_geometry, params)
+    rho = torch.from_numpy(np.ascontiguousarray(density)).to(dg.device).reshape(-1)
+    corr = plan.corrections(dg, rho)
+    records = tuple(
+        AtomicLocalPotentialPatchCorrection(p.atom_index, p.element, p.position, p.radius, p.shape,
+                                            float(params.correction_strength), main_e, patch_e, float(ce), p.nq)
+        for p, main_e, patch_e, ce, _, _ in corr)
+    total = float(sum(r.correction_energy for r in records))
+    return LocalIonicPotentialPatchEvaluation(base_evaluation, density, params, records, float(uncorrected),
+                                              float(uncorrected + total), total)
+
+
+def evaluate_monitor_grid_local_ionic_potential_with_frozen_patch_field(case, grid_geometry, density_field, *,
+                                                                        patch_parameters=None,
+                                                                        patch_evaluation=None,
+                                                                        base_evaluation=None):
+    """pseudo/local.py:585-703 on the GPU."""
+    from .reductions import wdot
+
+    dg = device_grid(grid_geometry)
+    density = _patch_density(density_field, dg)
+    params = LocalPotentialPatchParameters() if patch_parameters is None else patch_parameters
+    if patch_evaluation is None:
+        patch_evaluation = evaluate_monitor_grid_local_ionic_potential_with_patch(
+            case, grid_geometry, density, patch_parameters=params, base_evaluation=base_evaluation)
+    base = patch_evaluation.base_evaluation
+    plan = patch_plan(dg, case, grid_geometry, patch_evaluation.patch_parameters)
+    rho = torch.from_numpy(np.ascontiguousarray(density)).to(dg.device).reshape(-1)
+    corr = []
+    for p, rec in zip(plan.patches, patch_evaluation.atomic_patch_corrections):
+        mg = p.main_grid(dg, rho).cpu().numpy()
+        corr.append((p, rec.main_grid_patch_energy, rec.patch_integrated_energy, rec.correction_energy,
+                     float(mg[1]), int(round(float(mg[2])))))
+    field, scalings = plan.embed(dg, corr)
+    embedded = wdot(dg, rho, field)
+    f = field.cpu().numpy().reshape(dg.shape)
+    return FrozenPatchLocalPotentialEmbedding(
+        base, patch_evaluation, np.asarray(base.total_local_potential + f, dtype=np.float64), f, scalings,
+        float(embedded), float(embedded - patch_evaluation.total_patch_correction))
 
 
 # ------------------------------------------------------------------ nonlocal projectors

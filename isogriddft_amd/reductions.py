@@ -60,7 +60,7 @@ def wdot(dg, a: torch.Tensor, b: torch.Tensor | None, weighted: bool = True) -> 
 
 def gram(X: torch.Tensor, Y: torch.Tensor, w: torch.Tensor, out: torch.Tensor | None = None,
          accumulate: bool = False, stream=None) -> torch.Tensor:
-    """G = X W Y^T (FP64 result) for X (m, N), Y (n, N) of the same dtype."""
+    """G = X W Y^T (FP64 result) for X (m, N), Y (n, N) of the same dtype (w None: W = I)."""
     if X.dtype != Y.dtype:
         raise ValueError("X and Y must share a dtype")
     X2 = X.reshape(X.shape[0], -1)
@@ -69,7 +69,7 @@ def gram(X: torch.Tensor, Y: torch.Tensor, w: torch.Tensor, out: torch.Tensor | 
         raise ValueError("Gram operands must be row-contiguous")
     m, N = X2.shape
     n = Y2.shape[0]
-    if Y2.shape[1] != N or w.numel() != N:
+    if Y2.shape[1] != N or (w is not None and w.numel() != N):
         raise ValueError("weights must match the orbital grid size.")
     if out is None:
         out = torch.zeros((m, n), dtype=torch.float64, device=X.device)
@@ -77,10 +77,13 @@ def gram(X: torch.Tensor, Y: torch.Tensor, w: torch.Tensor, out: torch.Tensor | 
     need = lib.ig_gram_work_size(m, n, N)
     work = workspace(X.device, need, "gram")
     sfx = "f64" if X.dtype == torch.float64 else "f32"
-    ww = w.reshape(-1)
-    if ww.dtype != X.dtype:
-        ww = ww.to(X.dtype)
+    ww = None
+    if w is not None:
+        ww = w.reshape(-1)
+        if ww.dtype != X.dtype:
+            ww = ww.to(X.dtype)
+        ww = ww.contiguous()
     _lib.call(f"ig_gram_{sfx}", _lib.ptr(X2), X2.stride(0), m, _lib.ptr(Y2), Y2.stride(0), n,
-              _lib.ptr(ww.contiguous()), N, _lib.ptr(out), int(accumulate), _lib.ptr(work), work.numel(),
+              _lib.ptr(ww), N, _lib.ptr(out), int(accumulate), _lib.ptr(work), work.numel(),
               _lib.stream_handle(stream))
     return out

@@ -75,3 +75,69 @@ def test_single_point_components_match_oracle():
         assert abs(got.total - sum(ref.values())) <= 1e-8
         if "H2O" in name:
             assert abs(got.nonlocal_ionic) > 1e-6  # the O s-projector contributes
+
+
+def _axial_orbitals(geom, k):
+    """Orbitals on the H2 axis: the density keeps the x/y mirror symmetries of the grid,
+    so the patch neighbour choice among equidistant nodes does not matter."""
+    out = []
+    for n in range(k):
+        zc, a = (0.3, -0.5, 0.1)[n], (0.8, 0.6, 1.1)[n]
+        f = np.exp(-a * (geom.x_points ** 2 + geom.y_points ** 2 + (geom.z_points - zc) ** 2))
+        out.append(f / np.sqrt(np.sum(f * f * geom.cell_volumes)))
+    return np.array(out)
+
+
+@pytest.mark.parametrize("use_patch", [False, True])
+def test_static_local_energy_matches_oracle(small_monitor, use_patch):
+    """scf/driver.py:2668-2810: T_s (trial_fix) + E_loc (patch-corrected when on) + E_H +
+    E_xc + E_II.  E_H carries the reference's Poisson tolerance (rtol 1e-8): bar 1e-6 Ha
+    for it and the total, 1e-8 Ha for the rest."""
+    geom = small_monitor
+    up = _axial_orbitals(geom, 2)
+    dn = _axial_orbitals(geom, 1)
+    occ = energy.SpinOccupations(2.0, 1.0, (1.0, 1.0), (1.0,))
+    w = geom.cell_volumes
+    ru = oenergy.renormalize(oenergy.density(up, occ.occupations_up), 2.0, w)
+    rd = oenergy.renormalize(oenergy.density(dn, occ.occupations_down), 1.0, w)
+    got = energy.evaluate_static_local_single_point_energy(ru, rd, up, dn, occ, geom, ogrid.H2_CASE,
+                                                           use_monitor_patch=use_patch, kinetic_version="trial_fix")
+    rho = ru + rd
+    kin = sum(oenergy.expectation(p, okin.kinetic_operator(p, geom, "trial_fix"), w) for p in np.concatenate([up, dn]))
+    eloc = float(np.sum(rho * opseudo.local_ionic_potential(ogrid.H2_CASE, geom) * w))
+    if use_patch:
+        _, _, tot = opseudo.patch_local_correction_field(ogrid.H2_CASE, geom, rho, patch_radius_scale=0.75,
+                                                         correction_strength=1.3)
+        eloc += tot
+    pot, _, _, _ = opoisson.solve_poisson(geom, rho, order=2, method="direct")
+    eh = opoisson.hartree_energy(rho, geom, pot)
+    eps, _, _ = oxc.lsda_vwn(ru, rd)
+    exc = float(np.sum(rho * eps * w))
+    eii = oenergy.ion_ion(ogrid.H2_CASE)
+    ref = dict(kinetic=kin, local_ionic=eloc, hartree=eh, xc=exc, ion_ion_repulsion=eii)
+    for key, val in ref.items():
+        bar = 1e-6 if key == "hartree" else 1e-8
+        print(key, getattr(got, key), val)
+        assert abs(getattr(got, key) - val) <= bar, key
+    assert got.nonlocal_ionic == 0.0
+    assert abs(got.total - sum(ref.values())) <= 1e-6
+
+
+def test_static_local_energy_from_context(small_monitor):
+    from isogriddft_amd import eigensolver
+
+    geom = small_monitor
+    up = _axial_orbitals(geom, 1)
+    occ = energy.SpinOccupations(1.0, 1.0, (1.0,), (1.0,))
+    rho = up[0] ** 2
+    ctx = eigensolver.prepare_fixed_potential_static_local_operator(
+        geom, rho, rho, "up", ogrid.H2_CASE, use_monitor_patch=True,
+        patch_parameters=energy.default_monitor_patch_parameters(), kinetic_version="trial_fix")
+    assert ctx.frozen_patch_local_embedding is not None
+    got, prof = energy.evaluate_static_local_single_point_energy_from_context(ctx, up, up, occ)
+    direct = energy.evaluate_static_local_single_point_energy(rho, rho, up, up, occ, geom, ogrid.H2_CASE,
+                                                              use_monitor_patch=True, kinetic_version="trial_fix")
+    for key in ("kinetic", "local_ionic", "hartree", "xc", "ion_ion_repulsion", "total"):
+        print(key, getattr(got, key), getattr(direct, key))
+        assert abs(getattr(got, key) - getattr(direct, key)) <= 1e-8, key
+    assert prof.kinetic_wall_time_seconds >= 0.0

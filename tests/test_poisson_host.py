@@ -54,3 +54,24 @@ def test_quadrupole_is_traceless_three_m_minus_trace():
     M2 = np.einsum("p,pa,pb->ab", q, pts, pts)
     assert np.allclose(Q, opoisson.traceless_quadrupole(M2), rtol=0, atol=1e-13)
     assert abs(np.trace(Q)) < 1e-13
+
+
+def test_kinetic_diagonal_matches_assembled_operators():
+    """The eigensolver's Davidson preconditioner uses diag(Lap): exact for trial_fix and
+    legacy grids, and for the interior rows of the production closure."""
+    from isogriddft_amd import configs
+    from isogriddft_amd.device import DeviceGrid
+    from oracle import grid as ogrid
+    from oracle import kinetic as okin
+
+    g = configs.build_synthetic_geometry(shape=(10, 11, 12))
+    d = DeviceGrid(g, "cpu").kinetic_diagonal().numpy()
+    ref = okin.monitor_laplacian_matrix(g, "trial_fix").diagonal()
+    assert np.max(np.abs(d - ref)) <= 1e-14 * np.max(np.abs(ref))
+    inner = np.zeros(g.x_points.shape, bool)
+    inner[2:-2, 2:-2, 2:-2] = True
+    refp = okin.monitor_laplacian_matrix(g, "production").diagonal()
+    assert np.max(np.abs((d - refp)[inner.ravel()])) <= 1e-14 * np.max(np.abs(refp))
+    lg = ogrid.build_legacy_geometry(shape=(9, 10, 11))
+    dl = DeviceGrid(lg, "cpu").kinetic_diagonal().numpy()
+    assert np.array_equal(dl, okin.legacy_laplacian_matrix(lg).diagonal())
