@@ -1,21 +1,26 @@
 """Fixed-potential block eigensolver on the GPU (drop-in for the reference's
 ks/eigensolver.py entry points and the block iteration of ks/eigensolver_jax.py).
 
-Algorithm (ks/eigensolver_jax.py:417-524, restated on device-resident blocks):
-  X <- weighted QR(X0);  HX <- H X
+Algorithm (ks/eigensolver_jax.py:417-524, restated on device-resident blocks and in the
+W^(1/2)-similar form Z = W^(1/2) X, where W-inner products are plain ones and
+S = W^(1/2) H W^(-1/2) is the operator):
+  Z <- orthonormal(W^(1/2) X0);  HZ <- S Z
   repeat:
-    P = X^T W HX (symmetrised), (theta, V) = eigh(P); C = V[:, :k]
-    Psi = C^T X, HPsi = C^T HX, R = HPsi - theta Psi, r = ||R||_W
-    stop when max r <= tol
-    R <- (diag(H) - theta)^-1 R                             (Davidson diagonal preconditioner)
-    R <- R - (R^T W X) X, whitened by (R^T W R)^(-1/2)     (ks/eigensolver_jax.py:119-146)
-    HR = H R
-    T = [[P, X^T W HR], [R^T W HX, R^T W HR]], lowest m Ritz vectors [c1; c2]
-    X <- c1^T X + c2^T R, HX <- c1^T HX + c2^T HR, both whitened by (X^T W X)^(-1/2)
-Device work per iteration: one H apply of k orbitals (the fused HIP stencil), the
-weighted Gram products X^T W Y on MFMA (FP32 blocks) or FP64 FMAs (FP64 blocks) with
-FP64 accumulation (ig_gram_*), the basis rotations as library GEMMs (torch.matmul ->
-hipBLASLt), and the small FP64 eigendecompositions (torch.linalg.eigh on the GPU).
+    Ritz pairs of span(Z): (theta, C) from P = Z HZ^T and G = Z Z^T, measured every 16
+      iterations; in between the basis already is the Ritz basis (C = I, theta from the
+      previous step)
+    Psi = C^T Z, R = C^T HZ - theta Psi;  stop when max ||R|| <= tol
+    R <- (diag(H) - theta)^-1 R                  (Davidson diagonal preconditioner)
+    R <- R - (R Z^T) Z, whitened by the Cholesky factor of R R^T
+                                                 (ks/eigensolver_jax.py:119-146)
+    HR = S R
+    generalized Rayleigh-Ritz on [Z; R]: T c = theta G_B c with T = [[P, Z HR^T],
+      [R HZ^T, R HR^T]], G_B the Gram matrix of [Z; R]; lowest m Ritz vectors c
+    Z <- c^T [Z; R], HZ <- c^T [HZ; HR]
+Device work per iteration: one H apply of k orbitals (the fused HIP stencil), plain
+products (library GEMMs for wide outputs, the split-K Gram kernel for narrow ones) and the
+basis rotations as library GEMMs; the (m + k)-sized generalized eigenproblems run on the
+host (LAPACK).
 
 The preconditioner is an addition to the reference's block iteration (which has none):
 it changes the iterates, not the Ritz pairs converged to, and cuts the iteration count
@@ -118,11 +123,14 @@ class BlockIterationResult:
     timings: dict = field(default_factory=dict)
 
 
-def _host_eigh(A):
-    """Symmetric eigendecomposition of a small FP64 matrix on the host (LAPACK): for the
-    200-600 wide projected matrices it is 2-3x faster than the device solver."""
-    ev, V = torch.linalg.eigh(A.detach().to("cpu", torch.float64))
-    return ev, V
+def _host_geigh(A, B):
+    """Generalized symmetric eigenproblem A c = theta B c (B SPD) on the host: ascending
+    theta and c with c^T B c = I, by Cholesky reduction."""
+    Ah = A.detach().to("cpu", torch.float64)
+    Linv = _chol_inv(B)
+    At = Linv @ Ah @ Linv.T
+    ev, Y = torch.linalg.eigh(0.5 * (At + At.T))
+    return ev, Linv.T @ Y
 
 
 def _chol_inv(G):
@@ -216,38 +224,45 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
     apply_s(ZH[0], ZH[1])
     RH = torch.empty((2, k, N), dtype=dt, device=dev)  # RH[0] = R, RH[1] = S R
 
-    eye_m = torch.eye(m, dtype=torch.float64, device=dev)
     res = BlockIterationResult(np.zeros(k), ZH[0, :k], np.full(k, np.inf), 0, False)
     phase32 = dt == torch.float32
     Psi = None
     it = 0
+    # After each Rayleigh-Ritz step the basis is the Ritz basis of the (m + k) space:
+    # Z HZ^T = diag(theta_all), Z Z^T = I.  Iterations in between re-measurements take
+    # the Ritz vectors and values from there; every 16 iterations (and after the FP32 ->
+    # FP64 promotion) both matrices are measured and the small eigenproblem solved.
+    theta_all = None
+    refresh = True
     for it in range(1, max_iter + 1):
         Z, HZ = ZH[0], ZH[1]
-        mark("rotation")
-        if it % 16 == 1:
-            # re-measure the basis Gram every 16 iterations (between, the Cholesky whitening
-            # keeps it the identity to working precision)
-            G1 = gemm_nt(Z, ZH.view(2 * m, N))  # rows of ZH = [Z; HZ]  ->  [Z Z^T | Z HZ^T]
-            Gzz = G1[:, :m]
-            P = 0.5 * (G1[:, m:] + G1[:, m:].T)
-        else:
-            Gzz = eye_m
-            P = gemm_nt(Z, HZ)
-            P = 0.5 * (P + P.T)
-        mark("gram")
-        theta, V = _host_eigh(P)
-        mark("eigh")
-        C = V[:, :k].to(dev)
-        Ct = C.T.to(dt).contiguous()
-        Psi = torch.matmul(Ct, Z)
         R = RH[0]
-        torch.matmul(Ct, HZ, out=R)
-        th = theta[:k].numpy()
-        R.sub_(torch.from_numpy(th).to(dev, dt)[:, None] * Psi)
+        mark("rotation")
+        if refresh or it % 16 == 1:
+            G1 = gemm_nt(Z, ZH.view(2 * m, N))  # rows of ZH = [Z; HZ]  ->  [Z Z^T | Z HZ^T]
+            Gzz = _sym(G1[:, :m])
+            P = _sym(G1[:, m:])
+            mark("gram")
+            theta, V = _host_geigh(P, Gzz)
+            mark("eigh")
+            Ct = V[:, :k].T.to(dev, dt).contiguous()
+            Psi = torch.matmul(Ct, Z)
+            torch.matmul(Ct, HZ, out=R)
+            th = theta[:k].numpy()
+            R.sub_(torch.from_numpy(th).to(dev, dt)[:, None] * Psi)
+            refresh = False
+        else:
+            Gzz = None
+            P = None
+            th = theta_all[:k]
+            Psi = Z[:k]
+            torch.mul(Psi, torch.from_numpy(th).to(dev, dt)[:, None], out=R)
+            torch.sub(HZ[:k], R, out=R)
         rn = torch.linalg.vector_norm(R.to(torch.float64) if dt == torch.float32 else R, dim=1).cpu().numpy()
         res.residual_history.append(rn)
         res.ritz_value_history.append(th)
         res.subspace_dimensions.append(m)
+        ritz_k = P[:k, :k].cpu().numpy() if P is not None else np.diag(th)
         target = max(tol, fp32_floor) if phase32 else tol
         # FP32 stops helping once the residual stagnates: no 5 % gain over 60 iterations
         stalled = phase32 and len(res.residual_history) > 60 and \
@@ -266,10 +281,11 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
                 del Z64
                 apply_s(ZH[0], ZH[1])
                 RH = torch.empty((2, k, N), dtype=dt, device=dev)
+                refresh = True
                 continue
             res.eigenvalues, res.orbitals, res.residual_norms = th, Psi, rn
             res.iterations, res.converged = it, True
-            res.ritz_matrix = P[:k, :k].cpu().numpy()
+            res.ritz_matrix = ritz_k
             break
         mark("rotation")
         if hdiag is not None:
@@ -290,15 +306,17 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
         G3 = gemm_nt(R, RH.view(2 * k, N))    # [R R^T | R HR^T]: (k, 2k)
         G4 = gemm_nt(R, HZ)                   # R HZ^T: (k, m)
         mark("gram")
-        ZR, ZHR = G2[:, :k], G2[:, k:]
-        RR, RHR = G3[:, :k], G3[:, k:]
-        T = torch.cat([torch.cat([P, ZHR], 1), torch.cat([G4, RHR], 1)], 0)
-        T = 0.5 * (T + T.T)
-        _, TV = _host_eigh(T)
-        c = TV[:, :m]
-        GB = torch.cat([torch.cat([0.5 * (Gzz + Gzz.T), ZR], 1), torch.cat([ZR.T, 0.5 * (RR + RR.T)], 1)], 0).cpu()
-        Gn = c.T @ GB @ c
-        Q = (_chol_inv(Gn) @ c.T).to(dev)  # new basis = Q [Z; R]
+        ZR, ZHR = G2[:, :k].cpu(), G2[:, k:].cpu()
+        RR, RHR = G3[:, :k].cpu(), G3[:, k:].cpu()
+        G4 = G4.cpu()
+        Pz = P.cpu() if P is not None else torch.diag(torch.from_numpy(theta_all))
+        Bz = Gzz.cpu() if Gzz is not None else torch.eye(m, dtype=torch.float64)
+        T = _sym(torch.cat([torch.cat([Pz, ZHR], 1), torch.cat([G4, RHR], 1)], 0))
+        GB = _sym(torch.cat([torch.cat([Bz, ZR], 1), torch.cat([ZR.T, RR], 1)], 0))
+        # generalized Rayleigh-Ritz on [Z; R]: c^T GB c = I, c^T T c = diag(theta)
+        thT, c = _host_geigh(T, GB)
+        theta_all = thT[:m].numpy().copy()
+        Q = c[:, :m].T.contiguous().to(dev)  # new basis = Q [Z; R]
         mark("eigh")
         Q1 = Q[:, :m].to(dt).contiguous()
         Q2 = Q[:, m:].to(dt).contiguous()
@@ -309,7 +327,7 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
         ZH, ZH_next = ZH_next, ZH
         res.eigenvalues, res.orbitals, res.residual_norms = th, Psi, rn
         res.iterations = it
-        res.ritz_matrix = P[:k, :k].cpu().numpy()
+        res.ritz_matrix = ritz_k
     mark("rotation")
     res.timings = dict(tm)
     sw, isw = setup(dt)
