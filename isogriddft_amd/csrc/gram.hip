@@ -130,15 +130,239 @@ __global__ __launch_bounds__(IG_THREADS) void gram_f64_simt(const double* __rest
     }
 }
 
-// G (+)= sum over partials; symmetrize optionally handled on the host.
+// G (+)= sum over partials; symmetrize optionally handled on the host.  With tcols > 0
+// the partials hold the transpose (rows of tcols entries) and G is written transposed.
 __global__ __launch_bounds__(IG_THREADS) void gram_reduce(const double* __restrict__ work, int nparts, long long mn,
-                                                          double* __restrict__ G, int accumulate) {
+                                                          double* __restrict__ G, int accumulate, int tcols = 0,
+                                                          int trows = 0) {
   for (long long t = (long long)blockIdx.x * IG_THREADS + threadIdx.x; t < mn;
        t += (long long)gridDim.x * IG_THREADS) {
     double s = 0.0;
     for (int p = 0; p < nparts; ++p) s += work[(long long)p * mn + t];
-    G[t] = accumulate ? G[t] + s : s;
+    // partial entry t = (i, j) of the transposed product -> G[j][i]
+    const long long g = tcols > 0 ? (t % tcols) * trows + t / tcols : t;
+    G[g] = accumulate ? G[g] + s : s;
   }
+}
+
+// ---------------------------------------------------------------------------------
+// Narrow products (m <= 256 rows of X, n <= 64 rows of Y): the whole m x n output fits
+// one workgroup, so the grid splits only K.  Thread (tm, tn) = (tid / 8, tid % 8) owns
+// rows tm + 32 a (a < 8) and columns tn + 8 b (b < RN) in registers; a K tile of 16 is
+// staged through LDS (K-major, FP64) with the next tile's loads in flight in registers.
+// Each pass over the operands is HBM-bound; the FP64 partials per K chunk are summed by
+// gram_reduce in a fixed order (deterministic).
+constexpr int NW_KT = 16, NW_RM = 8, NW_MMAX = 32 * NW_RM, NW_NMAX = 64;
+constexpr int NW_XP = NW_MMAX * NW_KT / 2 / IG_THREADS;  // X pairs per thread and tile (max)
+constexpr int NW_YP = NW_NMAX * NW_KT / 2 / IG_THREADS;  // Y pairs per thread and tile (max)
+
+template <typename T>
+struct Pair;
+template <>
+struct Pair<float> {
+  typedef float t __attribute__((ext_vector_type(2)));
+};
+template <>
+struct Pair<double> {
+  typedef double t __attribute__((ext_vector_type(2)));
+};
+
+template <typename T, int RN, bool PAIR>
+__global__ __launch_bounds__(IG_THREADS) void gram_narrow(const T* __restrict__ X, long long ldx, int m,
+                                                          const T* __restrict__ Y, long long ldy, int n,
+                                                          const T* __restrict__ w, long long N, long long kchunk,
+                                                          double* __restrict__ work) {
+  using PT = typename Pair<T>::t;
+  __shared__ double sx[NW_KT][NW_MMAX];
+  __shared__ double sy[NW_KT][NW_NMAX];
+  const int tid = threadIdx.x;
+  const int tm = tid >> 3, tn = tid & 7;
+  const long long kbeg = (long long)blockIdx.x * kchunk;
+  const long long kend = min(kbeg + kchunk, N);
+  double acc[NW_RM][RN];
+#pragma unroll
+  for (int a = 0; a < NW_RM; ++a)
+#pragma unroll
+    for (int b = 0; b < RN; ++b) acc[a][b] = 0.0;
+  // staging (PAIR): element pair q of a tile is (row q / 8, k = 2 (q % 8)); pairs never
+  // straddle the chunk end because chunks and N are even (the host guarantees both).
+  // Otherwise element q is (row q / 16, k = q % 16), one scalar load each.
+  PT px[NW_XP], py[NW_YP];
+  T qx[2 * NW_XP], qy[2 * NW_YP];
+  auto fetch_scalar = [&](long long k0) {
+#pragma unroll
+    for (int r = 0; r < 2 * NW_XP; ++r) {
+      const int q = tid + r * IG_THREADS;
+      const int row = q >> 4;
+      const long long k = k0 + (q & 15);
+      T v = T(0);
+      if (row < m && k < kend) {
+        v = X[(long long)row * ldx + k];
+        if (w) v *= w[k];
+      }
+      qx[r] = v;
+    }
+#pragma unroll
+    for (int r = 0; r < 2 * NW_YP; ++r) {
+      const int q = tid + r * IG_THREADS;
+      const int row = q >> 4;
+      const long long k = k0 + (q & 15);
+      qy[r] = (row < n && k < kend) ? Y[(long long)row * ldy + k] : T(0);
+    }
+  };
+  auto commit_scalar = [&]() {
+#pragma unroll
+    for (int r = 0; r < 2 * NW_XP; ++r) {
+      const int q = tid + r * IG_THREADS;
+      sx[q & 15][q >> 4] = (double)qx[r];
+    }
+#pragma unroll
+    for (int r = 0; r < 2 * NW_YP; ++r) {
+      const int q = tid + r * IG_THREADS;
+      sy[q & 15][q >> 4] = (double)qy[r];
+    }
+  };
+  auto fetch = [&](long long k0) {
+    if (!PAIR) {
+      fetch_scalar(k0);
+      return;
+    }
+#pragma unroll
+    for (int r = 0; r < NW_XP; ++r) {
+      const int q = tid + r * IG_THREADS;
+      const int row = q >> 3;
+      const long long k = k0 + 2 * (q & 7);
+      PT v = PT(0);
+      if (row < m && k < kend) {
+        v = *reinterpret_cast<const PT*>(X + (long long)row * ldx + k);
+        if (w) {
+          const PT ww = *reinterpret_cast<const PT*>(w + k);
+          v = v * ww;
+        }
+      }
+      px[r] = v;
+    }
+#pragma unroll
+    for (int r = 0; r < NW_YP; ++r) {
+      const int q = tid + r * IG_THREADS;
+      const int row = q >> 3;
+      const long long k = k0 + 2 * (q & 7);
+      PT v = PT(0);
+      if (row < n && k < kend) v = *reinterpret_cast<const PT*>(Y + (long long)row * ldy + k);
+      py[r] = v;
+    }
+  };
+  auto commit = [&]() {
+    if (!PAIR) {
+      commit_scalar();
+      return;
+    }
+#pragma unroll
+    for (int r = 0; r < NW_XP; ++r) {
+      const int q = tid + r * IG_THREADS;
+      const int row = q >> 3, kk = 2 * (q & 7);
+      if (row < NW_MMAX) {
+        sx[kk][row] = (double)px[r][0];
+        sx[kk + 1][row] = (double)px[r][1];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NW_YP; ++r) {
+      const int q = tid + r * IG_THREADS;
+      const int row = q >> 3, kk = 2 * (q & 7);
+      if (row < NW_NMAX) {
+        sy[kk][row] = (double)py[r][0];
+        sy[kk + 1][row] = (double)py[r][1];
+      }
+    }
+  };
+  fetch(kbeg);
+  for (long long k0 = kbeg; k0 < kend; k0 += NW_KT) {
+    __syncthreads();  // the previous tile has been consumed
+    commit();
+    __syncthreads();
+    if (k0 + NW_KT < kend) fetch(k0 + NW_KT);
+#pragma unroll 4
+    for (int kk = 0; kk < NW_KT; ++kk) {
+      double xa[NW_RM], yb[RN];
+#pragma unroll
+      for (int a = 0; a < NW_RM; ++a) xa[a] = sx[kk][tm + 32 * a];
+#pragma unroll
+      for (int b = 0; b < RN; ++b) yb[b] = sy[kk][tn + 8 * b];
+#pragma unroll
+      for (int a = 0; a < NW_RM; ++a)
+#pragma unroll
+        for (int b = 0; b < RN; ++b) acc[a][b] = fma(xa[a], yb[b], acc[a][b]);
+    }
+  }
+  double* out = work + (long long)blockIdx.x * m * n;
+#pragma unroll
+  for (int a = 0; a < NW_RM; ++a)
+#pragma unroll
+    for (int b = 0; b < RN; ++b) {
+      const int i = tm + 32 * a, j = tn + 8 * b;
+      if (i < m && j < n) out[(long long)i * n + j] = acc[a][b];
+    }
+}
+
+struct NarrowPlan {
+  bool use, swap, pair;
+  int nchunk;
+  long long kchunk;
+};
+
+NarrowPlan narrow_plan(int m, int n, long long N, long long ldx, long long ldy) {
+  NarrowPlan g{false, false, false, 0, 0};
+  const bool fits = n <= NW_NMAX && m <= NW_MMAX;
+  const bool fits_t = m <= NW_NMAX && n <= NW_MMAX;
+  if (!fits && !fits_t) return g;
+  g.use = true;
+  g.swap = !fits;
+  // paired (2-element) loads need even N and even row strides (pointer alignment is
+  // checked by the caller)
+  g.pair = !(N & 1) && !(ldx & 1) && !(ldy & 1);
+  long long nch = (N + 1023) / 1024;  // at least ~1024 K values per workgroup
+  if (nch > 512) nch = 512;
+  if (nch < 1) nch = 1;
+  long long kc = (N + nch - 1) / nch;
+  kc = ((kc + NW_KT - 1) / NW_KT) * NW_KT;
+  g.kchunk = kc;
+  g.nchunk = (int)((N + kc - 1) / kc);
+  return g;
+}
+
+template <typename T>
+int gram_narrow_launch(const NarrowPlan& g, const T* X, long long ldx, int m, const T* Y, long long ldy, int n,
+                       const T* w, long long N, double* G, int accumulate, double* work, hipStream_t s) {
+  // with swap the kernel computes Y W X^T (n x m) and the reduction writes it transposed
+  const T* A = g.swap ? Y : X;
+  const T* B = g.swap ? X : Y;
+  const long long lda = g.swap ? ldy : ldx, ldb = g.swap ? ldx : ldy;
+  const int ma = g.swap ? n : m, nb = g.swap ? m : n;
+  const dim3 grid(g.nchunk), block(IG_THREADS);
+#define IG_NARROW(RN_)                                                                                          \
+  do {                                                                                                          \
+    if (g.pair)                                                                                                 \
+      hipLaunchKernelGGL((gram_narrow<T, RN_, true>), grid, block, 0, s, A, lda, ma, B, ldb, nb, w, N, g.kchunk, \
+                         work);                                                                                 \
+    else                                                                                                        \
+      hipLaunchKernelGGL((gram_narrow<T, RN_, false>), grid, block, 0, s, A, lda, ma, B, ldb, nb, w, N,          \
+                         g.kchunk, work);                                                                       \
+  } while (0)
+  if (nb <= 8)
+    IG_NARROW(1);
+  else if (nb <= 16)
+    IG_NARROW(2);
+  else if (nb <= 32)
+    IG_NARROW(4);
+  else
+    IG_NARROW(8);
+#undef IG_NARROW
+  const long long mn = (long long)m * n;
+  hipLaunchKernelGGL(gram_reduce, dim3((unsigned)((mn + IG_THREADS - 1) / IG_THREADS)), dim3(IG_THREADS), 0, s,
+                     work, g.nchunk, mn, G, accumulate, g.swap ? m : 0, g.swap ? n : 0);
+  IG_CHECK_LAUNCH();
+  return IG_OK;
 }
 
 struct GramPlan {
@@ -170,7 +394,10 @@ extern "C" {
 long long ig_gram_work_size(int m, int n, long long N) {
   if (m <= 0 || n <= 0 || N <= 0) return 0;
   const GramPlan g = plan(m, n, N);
-  return (long long)g.nkb * m * n;
+  long long need = (long long)g.nkb * m * n;
+  const NarrowPlan nw = narrow_plan(m, n, N, 2, 2);
+  if (nw.use && (long long)nw.nchunk * m * n > need) need = (long long)nw.nchunk * m * n;
+  return need;
 }
 
 int ig_gram_f32(const float* X, long long ldx, int m, const float* Y, long long ldy, int n,
@@ -178,6 +405,14 @@ int ig_gram_f32(const float* X, long long ldx, int m, const float* Y, long long 
                 long long work_elems, hipStream_t s) {
   if (!X || !Y || !G || !work || m <= 0 || n <= 0 || N <= 0) return IG_EINVAL;
   if (ldx < N || ldy < N) return IG_EINVAL;
+  const NarrowPlan nw = narrow_plan(m, n, N, ldx, ldy);
+  if (nw.use && work_elems >= (long long)nw.nchunk * m * n) {
+    NarrowPlan g = nw;
+    const bool aligned = ((reinterpret_cast<uintptr_t>(X) | reinterpret_cast<uintptr_t>(Y) |
+                           reinterpret_cast<uintptr_t>(w)) & (2 * sizeof(float) - 1)) == 0;
+    g.pair = g.pair && aligned;
+    return gram_narrow_launch<float>(g, X, ldx, m, Y, ldy, n, w, N, G, accumulate, work, s);
+  }
   const GramPlan g = plan(m, n, N);
   if (work_elems < (long long)g.nkb * m * n) return IG_EINVAL;
   hipLaunchKernelGGL(gram_f32_mfma, dim3(g.tiles_m * g.tiles_n, g.nkb), dim3(IG_THREADS), 0, s, X, ldx, m,
@@ -194,6 +429,14 @@ int ig_gram_f64(const double* X, long long ldx, int m, const double* Y, long lon
                 long long work_elems, hipStream_t s) {
   if (!X || !Y || !G || !work || m <= 0 || n <= 0 || N <= 0) return IG_EINVAL;
   if (ldx < N || ldy < N) return IG_EINVAL;
+  const NarrowPlan nw = narrow_plan(m, n, N, ldx, ldy);
+  if (nw.use && work_elems >= (long long)nw.nchunk * m * n) {
+    NarrowPlan g = nw;
+    const bool aligned = ((reinterpret_cast<uintptr_t>(X) | reinterpret_cast<uintptr_t>(Y) |
+                           reinterpret_cast<uintptr_t>(w)) & (2 * sizeof(double) - 1)) == 0;
+    g.pair = g.pair && aligned;
+    return gram_narrow_launch<double>(g, X, ldx, m, Y, ldy, n, w, N, G, accumulate, work, s);
+  }
   const GramPlan g = plan(m, n, N);
   if (work_elems < (long long)g.nkb * m * n) return IG_EINVAL;
   hipLaunchKernelGGL(gram_f64_simt, dim3(g.tiles_m * g.tiles_n, g.nkb), dim3(IG_THREADS), 0, s, X, ldx, m,

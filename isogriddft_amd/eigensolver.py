@@ -204,11 +204,11 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
         mark("hamiltonian_apply")
 
     def gemm_nt(A, B):
-        """A B^T as FP64.  Wide products go to the library GEMM in the block dtype; narrow
-        ones (an output under 128 x 128: the residual blocks, small working subspaces)
-        to the split-K Gram kernel, which keeps the whole chip busy where a library
-        kernel without a K split runs on a few workgroups."""
-        if A.shape[0] * B.shape[0] < 128 * 128:
+        """A B^T as FP64.  Products with one side of at most 64 rows and the other of at
+        most 256 go to the split-K Gram kernels (a library GEMM without a K split runs
+        them on a few workgroups); wider ones to the library GEMM in the block dtype."""
+        a, b = A.shape[0], B.shape[0]
+        if min(a, b) <= 64 and max(a, b) <= 256:
             return gram(A, B, None)
         return torch.matmul(A, B.T).to(torch.float64)
 
@@ -217,14 +217,19 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
         Linv = _chol_inv(G).to(dev)
         return torch.matmul(Linv.to(Z.dtype), Z)
 
-    # ZH[0] = Z (basis), ZH[1] = S Z; two buffers alternate between iterations
-    ZH = torch.empty((2, m, N), dtype=dt, device=dev)
-    ZH_next = torch.empty_like(ZH)
-    ZH[0].copy_(orthonormal_start((X0.reshape(m, N).to(dt) * sw)))
-    apply_s(ZH[0], ZH[1])
-    RH = torch.empty((2, k, N), dtype=dt, device=dev)  # RH[0] = R, RH[1] = S R
+    # One buffer holds the rows [Z; R; HR; HZ] (basis, residuals, S R, S Z), so [Z; R],
+    # [R; HR] and [HR; HZ] are contiguous: each basis rotation is one GEMM and one pass
+    # of the Gram kernel gives Z R^T, Z HR^T, R R^T and R HR^T together.  Two buffers
+    # alternate between iterations.
+    def views(B):
+        return B[:m], B[m:m + k], B[m + k:m + 2 * k], B[m + 2 * k:]
 
-    res = BlockIterationResult(np.zeros(k), ZH[0, :k], np.full(k, np.inf), 0, False)
+    BUF = torch.empty((2 * m + 2 * k, N), dtype=dt, device=dev)
+    BUF_next = torch.empty_like(BUF)
+    BUF[:m].copy_(orthonormal_start((X0.reshape(m, N).to(dt) * sw)))
+    apply_s(BUF[:m], BUF[m + 2 * k:])
+
+    res = BlockIterationResult(np.zeros(k), BUF[:k], np.full(k, np.inf), 0, False)
     phase32 = dt == torch.float32
     Psi = None
     it = 0
@@ -235,13 +240,11 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
     theta_all = None
     refresh = True
     for it in range(1, max_iter + 1):
-        Z, HZ = ZH[0], ZH[1]
-        R = RH[0]
+        Z, R, HR, HZ = views(BUF)
         mark("rotation")
         if refresh or it % 16 == 1:
-            G1 = gemm_nt(Z, ZH.view(2 * m, N))  # rows of ZH = [Z; HZ]  ->  [Z Z^T | Z HZ^T]
-            Gzz = _sym(G1[:, :m])
-            P = _sym(G1[:, m:])
+            Gzz = _sym(gemm_nt(Z, Z))
+            P = _sym(gemm_nt(Z, HZ))
             mark("gram")
             theta, V = _host_geigh(P, Gzz)
             mark("eigh")
@@ -275,12 +278,12 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
                 dt = torch.float64
                 sw, isw = setup(dt)
                 Z64 = orthonormal_start(Z.to(torch.float64))
-                ZH = torch.empty((2, m, N), dtype=dt, device=dev)
-                ZH_next = torch.empty_like(ZH)
-                ZH[0].copy_(Z64)
+                del BUF_next
+                BUF = torch.empty((2 * m + 2 * k, N), dtype=dt, device=dev)
+                BUF_next = torch.empty_like(BUF)
+                BUF[:m].copy_(Z64)
                 del Z64
-                apply_s(ZH[0], ZH[1])
-                RH = torch.empty((2, k, N), dtype=dt, device=dev)
+                apply_s(BUF[:m], BUF[m + 2 * k:])
                 refresh = True
                 continue
             res.eigenvalues, res.orbitals, res.residual_norms = th, Psi, rn
@@ -300,15 +303,12 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
         Linv_r = _chol_inv(GR).to(dev)
         mark("eigh")
         R.copy_(torch.matmul(Linv_r.to(dt), R))
-        apply_s(R, RH[1])
-        HR = RH[1]
-        G2 = gemm_nt(Z, RH.view(2 * k, N))    # [Z R^T | Z HR^T]: (m, 2k)
-        G3 = gemm_nt(R, RH.view(2 * k, N))    # [R R^T | R HR^T]: (k, 2k)
-        G4 = gemm_nt(R, HZ)                   # R HZ^T: (k, m)
+        apply_s(R, HR)
+        G23 = gemm_nt(BUF[:m + k], BUF[m:m + 2 * k]).cpu()  # [Z; R] [R; HR]^T: (m + k, 2k)
+        G4 = gemm_nt(R, HZ).cpu()                             # R HZ^T: (k, m)
         mark("gram")
-        ZR, ZHR = G2[:, :k].cpu(), G2[:, k:].cpu()
-        RR, RHR = G3[:, :k].cpu(), G3[:, k:].cpu()
-        G4 = G4.cpu()
+        ZR, ZHR = G23[:m, :k], G23[:m, k:]
+        RR, RHR = G23[m:, :k], G23[m:, k:]
         Pz = P.cpu() if P is not None else torch.diag(torch.from_numpy(theta_all))
         Bz = Gzz.cpu() if Gzz is not None else torch.eye(m, dtype=torch.float64)
         T = _sym(torch.cat([torch.cat([Pz, ZHR], 1), torch.cat([G4, RHR], 1)], 0))
@@ -318,13 +318,11 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
         theta_all = thT[:m].numpy().copy()
         Q = c[:, :m].T.contiguous().to(dev)  # new basis = Q [Z; R]
         mark("eigh")
-        Q1 = Q[:, :m].to(dt).contiguous()
-        Q2 = Q[:, m:].to(dt).contiguous()
-        torch.matmul(Q1, Z, out=ZH_next[0])
-        ZH_next[0].addmm_(Q2, R)
-        torch.matmul(Q1, HZ, out=ZH_next[1])
-        ZH_next[1].addmm_(Q2, HR)
-        ZH, ZH_next = ZH_next, ZH
+        Qd = Q.to(dt)
+        torch.matmul(Qd, BUF[:m + k], out=BUF_next[:m])               # Z' = Q [Z; R]
+        Qh = torch.cat([Qd[:, m:], Qd[:, :m]], 1)
+        torch.matmul(Qh, BUF[m + k:], out=BUF_next[m + 2 * k:])        # HZ' = Q [HR; HZ] reordered
+        BUF, BUF_next = BUF_next, BUF
         res.eigenvalues, res.orbitals, res.residual_norms = th, Psi, rn
         res.iterations = it
         res.ritz_matrix = ritz_k
@@ -332,7 +330,7 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
     res.timings = dict(tm)
     sw, isw = setup(dt)
     if keep_basis:
-        res.basis = ZH[0] * isw
+        res.basis = BUF[:m] * isw
     res.orbitals = (res.orbitals * isw).reshape((k,) + shape)
     return res
 

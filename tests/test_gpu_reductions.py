@@ -50,3 +50,28 @@ def test_dots_multi():
     reductions.dots_into(out32, [(t[0].float(), t[1].float())], None)
     ref = np.sum(a.astype(np.float32).astype(np.float64) * b.astype(np.float32).astype(np.float64))
     assert abs(out32[0].item() - ref) < 1e-9 * np.sum(np.abs(a * b))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("m,n,N,weighted", [(216, 32, 221184, False), (16, 200, 40000, True), (256, 64, 1001, True),
+                                             (64, 256, 2048, False), (1, 1, 3, True), (200, 16, 132651, False)])
+def test_gram_narrow_shapes(dtype, m, n, N, weighted):
+    """The split-K narrow kernel (one side <= 64 rows, the other <= 256): both operand
+    orientations, paired and scalar loads (odd N), with and without weights, and the
+    accumulate flag.  FP32 inputs accumulate in FP64 (bar 1e-12 on the FP32 products)."""
+    rng = np.random.default_rng(m * 7 + n)
+    X = rng.normal(size=(m, N)).astype(dtype)
+    Y = rng.normal(size=(n, N)).astype(dtype)
+    w = rng.uniform(0.1, 2.0, size=N).astype(dtype) if weighted else None
+    # the weights are applied, in the input precision, to the operand the kernel stages as
+    # the wide side: X unless X has over 256 rows or Y over 64 (then the roles swap)
+    swapped = not (n <= 64 and m <= 256)
+    Xw = (X * w).astype(dtype) if weighted and not swapped else X
+    Yw = (Y * w).astype(dtype) if weighted and swapped else Y
+    ref = Xw.astype(np.float64) @ Yw.astype(np.float64).T
+    t = lambda a: None if a is None else torch.from_numpy(a).cuda()  # noqa: E731
+    G = reductions.gram(t(X), t(Y), t(w))
+    scale = np.sqrt(np.outer(np.sum(Xw.astype(np.float64) ** 2, 1), np.sum(Yw.astype(np.float64) ** 2, 1)))
+    assert np.max(np.abs(G.cpu().numpy() - ref) / scale) < 1e-12
+    G2 = reductions.gram(t(X), t(Y), t(w), out=G.clone(), accumulate=True)
+    assert np.max(np.abs(G2.cpu().numpy() - 2 * ref) / scale) < 1e-12
