@@ -130,20 +130,33 @@ __global__ __launch_bounds__(IG_THREADS) void gram_f64_simt(const double* __rest
     }
 }
 
-// G (+)= sum over partials; symmetrize optionally handled on the host.  With tcols > 0
-// the partials hold the transpose (rows of tcols entries) and G is written transposed.
+// G (+)= sum over partials.  A workgroup owns 32 consecutive entries; its 8 slices of 32
+// lanes each sum every 8th partial (coalesced 256-byte rows) and the slice sums are
+// combined in a fixed order (deterministic).  With tcols > 0 the partials hold the
+// transpose (rows of tcols entries) and G is written transposed.
+constexpr int GR_OUT = 32, GR_SL = IG_THREADS / GR_OUT;
 __global__ __launch_bounds__(IG_THREADS) void gram_reduce(const double* __restrict__ work, int nparts, long long mn,
                                                           double* __restrict__ G, int accumulate, int tcols = 0,
                                                           int trows = 0) {
-  for (long long t = (long long)blockIdx.x * IG_THREADS + threadIdx.x; t < mn;
-       t += (long long)gridDim.x * IG_THREADS) {
-    double s = 0.0;
-    for (int p = 0; p < nparts; ++p) s += work[(long long)p * mn + t];
+  __shared__ double part[GR_SL][GR_OUT];
+  const int tx = threadIdx.x % GR_OUT, sl = threadIdx.x / GR_OUT;
+  const long long t = (long long)blockIdx.x * GR_OUT + tx;
+  double s = 0.0;
+  if (t < mn)
+    for (int p = sl; p < nparts; p += GR_SL) s += work[(long long)p * mn + t];
+  part[sl][tx] = s;
+  __syncthreads();
+  if (sl == 0 && t < mn) {
+    double tot = part[0][tx];
+#pragma unroll
+    for (int q = 1; q < GR_SL; ++q) tot += part[q][tx];
     // partial entry t = (i, j) of the transposed product -> G[j][i]
     const long long g = tcols > 0 ? (t % tcols) * trows + t / tcols : t;
-    G[g] = accumulate ? G[g] + s : s;
+    G[g] = accumulate ? G[g] + tot : tot;
   }
 }
+
+inline unsigned reduce_blocks(long long mn) { return (unsigned)((mn + GR_OUT - 1) / GR_OUT); }
 
 // ---------------------------------------------------------------------------------
 // Narrow products (m <= 256 rows of X, n <= 64 rows of Y): the whole m x n output fits
@@ -321,8 +334,8 @@ NarrowPlan narrow_plan(int m, int n, long long N, long long ldx, long long ldy) 
   // paired (2-element) loads need even N and even row strides (pointer alignment is
   // checked by the caller)
   g.pair = !(N & 1) && !(ldx & 1) && !(ldy & 1);
-  long long nch = (N + 1023) / 1024;  // at least ~1024 K values per workgroup
-  if (nch > 512) nch = 512;
+  long long nch = (N + 511) / 512;  // at least ~512 K values per workgroup
+  if (nch > 1024) nch = 1024;
   if (nch < 1) nch = 1;
   long long kc = (N + nch - 1) / nch;
   kc = ((kc + NW_KT - 1) / NW_KT) * NW_KT;
@@ -359,8 +372,8 @@ int gram_narrow_launch(const NarrowPlan& g, const T* X, long long ldx, int m, co
     IG_NARROW(8);
 #undef IG_NARROW
   const long long mn = (long long)m * n;
-  hipLaunchKernelGGL(gram_reduce, dim3((unsigned)((mn + IG_THREADS - 1) / IG_THREADS)), dim3(IG_THREADS), 0, s,
-                     work, g.nchunk, mn, G, accumulate, g.swap ? m : 0, g.swap ? n : 0);
+  hipLaunchKernelGGL(gram_reduce, dim3(reduce_blocks(mn)), dim3(IG_THREADS), 0, s, work, g.nchunk, mn, G, accumulate,
+                     g.swap ? m : 0, g.swap ? n : 0);
   IG_CHECK_LAUNCH();
   return IG_OK;
 }
@@ -418,8 +431,7 @@ int ig_gram_f32(const float* X, long long ldx, int m, const float* Y, long long 
   hipLaunchKernelGGL(gram_f32_mfma, dim3(g.tiles_m * g.tiles_n, g.nkb), dim3(IG_THREADS), 0, s, X, ldx, m,
                      Y, ldy, n, w, N, g.kchunk, g.tiles_n, work);
   const long long mn = (long long)m * n;
-  hipLaunchKernelGGL(gram_reduce, dim3((unsigned)((mn + IG_THREADS - 1) / IG_THREADS)), dim3(IG_THREADS), 0, s,
-                     work, g.nkb, mn, G, accumulate);
+  hipLaunchKernelGGL(gram_reduce, dim3(reduce_blocks(mn)), dim3(IG_THREADS), 0, s, work, g.nkb, mn, G, accumulate, 0, 0);
   IG_CHECK_LAUNCH();
   return IG_OK;
 }
@@ -442,8 +454,7 @@ int ig_gram_f64(const double* X, long long ldx, int m, const double* Y, long lon
   hipLaunchKernelGGL(gram_f64_simt, dim3(g.tiles_m * g.tiles_n, g.nkb), dim3(IG_THREADS), 0, s, X, ldx, m,
                      Y, ldy, n, w, N, g.kchunk, g.tiles_n, work);
   const long long mn = (long long)m * n;
-  hipLaunchKernelGGL(gram_reduce, dim3((unsigned)((mn + IG_THREADS - 1) / IG_THREADS)), dim3(IG_THREADS), 0, s,
-                     work, g.nkb, mn, G, accumulate);
+  hipLaunchKernelGGL(gram_reduce, dim3(reduce_blocks(mn)), dim3(IG_THREADS), 0, s, work, g.nkb, mn, G, accumulate, 0, 0);
   IG_CHECK_LAUNCH();
   return IG_OK;
 }
