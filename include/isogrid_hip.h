@@ -260,6 +260,37 @@ int ig_patch_main_f64(const double* xyz, const double* rho, const double* w, lon
 int ig_patch_field_f64(const double* xyz, long long N, const double* atom, double radius,
                        int constant, double scale, double* out, hipStream_t stream);
 
+/* ---- monitor-grid generator (grid/monitor_geometry.py:51-577) --------------------- */
+
+/* Doubles of `work` needed by the sweep and Jacobian reductions below. */
+int ig_grid_work_size(void);
+/* Atom-centred monitor 1 + sum_A cap (1 - exp(-raw_A / cap)) on the points xyz[3*N] (SoA);
+ * atoms = natoms x 13 doubles (x, y, z, kinetic weight / radius / exponent, local weight /
+ * radius / exponent, projector weight / radius / exponent, cap), device memory.
+ * Replaces: grid/monitor_geometry.py:51-112 (evaluate_global_monitor_field, values). */
+int ig_monitor_field_f64(const double* xyz, long long N, const double* atoms, int natoms, double* out,
+                         hipStream_t stream);
+/* out = max((1 - s) m + s avg7(m), 1) inside, max(m, 1) on the faces (s > 0); out = m for
+ * s <= 0.  out must not alias m.  Replaces: grid/monitor_geometry.py:115-136. */
+int ig_monitor_smooth_f64(const double* mon, int nx, int ny, int nz, double smoothing, double* out,
+                          hipStream_t stream);
+/* One damped Jacobi sweep of div(M grad X) = 0 for the SoA coordinates (faces copied):
+ * out = (1 - relax) X + relax X_jacobi inside; *max_change (device) = max |out - X| over
+ * the interior.  dx, dy, dz: logical spacings.  Replaces: one inner iteration of
+ * grid/monitor_geometry.py:268-321 (_solve_weighted_harmonic_coordinates). */
+int ig_harmonic_sweep_f64(const double* mon, const double* coords, double* out, int nx, int ny, int nz,
+                          double dx, double dy, double dz, double relax, double* work, double* max_change,
+                          hipStream_t stream);
+/* *min_jac (device) = min det(d x / d xi) of the SoA coordinates (np.gradient, edge order 2).
+ * Replaces: the check of grid/monitor_geometry.py:376-396 (_backtracking_update). */
+int ig_mapping_min_jacobian_f64(const double* coords, int nx, int ny, int nz, double hx, double hy,
+                                double hz, double* work, double* min_jac, hipStream_t stream);
+/* Jacobian, inverse metric tensor (9 per point, row-major) and cell volumes J hx hy hz of a
+ * mapped grid.  Replaces: grid/monitor_geometry.py:324-373. */
+int ig_mapping_metric_f64(const double* coords, int nx, int ny, int nz, double hx, double hy, double hz,
+                          double* jacobian, double* inverse_metric, double* cell_volumes,
+                          hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

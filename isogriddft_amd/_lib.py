@@ -61,6 +61,12 @@ SIGNATURES = {
     "ig_idw_quadrature_f64": [P, P, P, I, I, P, P, P, P, P],
     "ig_patch_main_f64": [P, P, P, L, P, D, P, P, P],
     "ig_patch_field_f64": [P, L, P, D, I, D, P, P],
+    "ig_grid_work_size": [],
+    "ig_monitor_field_f64": [P, L, P, I, P, P],
+    "ig_monitor_smooth_f64": [P, I, I, I, D, P, P],
+    "ig_harmonic_sweep_f64": [P, P, P, I, I, I, D, D, D, D, P, P, P],
+    "ig_mapping_min_jacobian_f64": [P, I, I, I, D, D, D, P, P, P],
+    "ig_mapping_metric_f64": [P, I, I, I, D, D, D, P, P, P, P],
 }
 _RESTYPE = {"ig_gram_work_size": L, "ig_knn_work_bytes": L}
 
