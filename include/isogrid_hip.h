@@ -229,6 +229,20 @@ int ig_cell_split_moments_f64(const double* rho, const double* xyz, const double
                               int nz, const int* cells, int ncells, const double* coef, double subvol,
                               const double* center, int order, double* work, double* out,
                               hipStream_t stream);
+/* Adjoint of the nodal multipole moments: out (=|+=) w (1 - frac) sum_j q_j mono_j(x - c)
+ * (frac may be NULL; q: nmono(order) host doubles, center: host).  No reference
+ * counterpart (the reference has no derivatives): the transpose of ig_multipole_moments_f64,
+ * used by the Poisson adjoint. */
+int ig_moments_adjoint_f64(const double* w, const double* frac, const double* xyz, long long N,
+                           const double* center, int order, const double* q, double* out, int accumulate,
+                           hipStream_t stream);
+/* Adjoint of the sub-cell quadratic-fit moments (ig_cell_fit_f64 + ig_cell_moments_f64):
+ * out += sum_j q_j d moment_j / d rho, scattered onto the 4x4x4 fit stencils (atomic adds).
+ * No reference counterpart: the transpose of the corrected-moment construction
+ * (poisson/open_boundary.py:370-464), used by the Poisson adjoint. */
+int ig_cell_moments_adjoint_f64(const double* xyz, const double* jac, int nx, int ny, int nz,
+                                const int* cells, int ncells, double subvol, const double* center,
+                                int order, const double* q, double* out, hipStream_t stream);
 
 /* ---- near-core local-GTH patch (pseudo/local.py:379-703) ------------------------- */
 

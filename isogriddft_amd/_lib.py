@@ -56,6 +56,8 @@ SIGNATURES = {
     "ig_cell_fit_f64": [P, P, I, I, I, P, I, P, P],
     "ig_cell_moments_f64": [P, P, I, I, I, P, I, P, D, P, I, P, P, P],
     "ig_cell_split_moments_f64": [P, P, P, I, I, I, P, I, P, D, P, I, P, P, P],
+    "ig_moments_adjoint_f64": [P, P, P, L, P, I, P, P, I, P],
+    "ig_cell_moments_adjoint_f64": [P, P, I, I, I, P, I, D, P, I, P, P, P],
     "ig_knn_work_bytes": [I, I],
     "ig_knn_f64": [P, L, P, I, I, P, L, P, P, P],
     "ig_idw_quadrature_f64": [P, P, P, I, I, P, P, P, P, P],

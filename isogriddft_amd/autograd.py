@@ -14,6 +14,10 @@ on the same HIP kernels as the forward path:
                       vectors normalised under the weights w that make T + V self-adjoint
                       (cell volumes on monitor grids, (h s_x)(h s_y)(h s_z) on legacy
                       grids), i.e. dE/dv_ext for E = sum_i g_i eps_i.
+* ``HartreePotential`` v_H = P rho (open-boundary Poisson); backward is the adjoint
+                      solve P^T g (transposed stencil in BiCGSTAB, transposed multipole
+                      faces and moments), giving dE_H / d rho and the Hartree part of
+                      dE/dv_ext chains.
 * ``IonicEnergy``     E_ion(R) = E_loc + E_nl + E_II at frozen density and orbitals as a
                       function of the atomic positions; backward gives dE/dR (minus the
                       Hellmann-Feynman forces): ig_vloc_gth_grad_f64 for the local part,
@@ -91,6 +95,40 @@ class RitzValues(torch.autograd.Function):
 
 def ritz_values(veff, dg, X0, k, tol=1e-8, closure="trial_fix"):
     return RitzValues.apply(veff, dg, X0, k, tol, closure)
+
+
+class HartreePotential(torch.autograd.Function):
+    """v_H = P rho, the open-boundary Poisson map (multipole faces + interior solve);
+    backward: P^T g by the adjoint solve of poisson.poisson_adjoint_device (BiCGSTAB on
+    the transposed stencil, recomputed rather than stored)."""
+
+    @staticmethod
+    def forward(ctx, rho, dg: DeviceGrid, multipole_order=2, mode="corrected_moments", tol=1e-10):
+        from .poisson import solve_poisson_device
+
+        sol = solve_poisson_device(dg, rho.detach().reshape(-1).to(torch.float64), multipole_order=multipole_order,
+                                   tolerance=tol, max_iterations=4000, mode=mode)
+        ctx.dg, ctx.order, ctx.mode, ctx.tol = dg, multipole_order, mode, tol
+        return sol["potential"].reshape(rho.shape)
+
+    @staticmethod
+    def backward(ctx, g):
+        from .poisson import poisson_adjoint_device
+
+        grad, info = poisson_adjoint_device(ctx.dg, g, ctx.order, ctx.mode, ctx.tol)
+        if not info["converged"]:
+            raise RuntimeError(f"the adjoint Poisson solve did not converge: {info}")
+        return grad.reshape(g.shape), None, None, None, None
+
+
+def hartree_potential(rho, dg, multipole_order=2, mode="corrected_moments", tol=1e-10):
+    return HartreePotential.apply(rho, dg, multipole_order, mode, tol)
+
+
+def hartree_energy(rho, dg, multipole_order=2, mode="corrected_moments", tol=1e-10):
+    """E_H = 1/2 sum w rho v_H[rho], differentiable in rho (dE_H / d rho)."""
+    v = hartree_potential(rho, dg, multipole_order, mode, tol)
+    return 0.5 * (dg.weights() * rho.reshape(-1) * v.reshape(-1)).sum()
 
 
 def _ion_ion(elements, R):

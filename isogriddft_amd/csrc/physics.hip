@@ -601,6 +601,172 @@ inline unsigned grid1d(long long n) {
   return (unsigned)b;
 }
 
+// ------------------------------------------------------------------ moment adjoints
+// The multipole moments are linear functionals of rho; their adjoints map a vector q of
+// moment weights back to a field: out = sum_j q_j d moment_j / d rho.
+
+struct MomentWeights {
+  double q[35];
+};
+
+// Nodal moments sum_p w_p (1 - frac_p) rho_p mono_j(x_p - c)  (frac may be NULL).
+template <int ORDER>
+__global__ __launch_bounds__(IG_THREADS) void moments_adjoint_kernel(const double* __restrict__ w,
+                                                                     const double* __restrict__ frac,
+                                                                     const double* __restrict__ xyz, long long N,
+                                                                     double cx, double cy, double cz, MomentWeights q,
+                                                                     double* __restrict__ out, int accumulate) {
+  constexpr int NM = nmono(ORDER);
+  for (long long t = (long long)blockIdx.x * IG_THREADS + threadIdx.x; t < N; t += (long long)gridDim.x * IG_THREADS) {
+    double mm[NM];
+    monomials<ORDER>(xyz[t] - cx, xyz[N + t] - cy, xyz[2 * N + t] - cz, mm);
+    double sum = 0.0;
+#pragma unroll
+    for (int i = 0; i < NM; ++i) sum += q.q[i] * mm[i];
+    const double v = w[t] * (frac ? 1.0 - frac[t] : 1.0) * sum;
+    out[t] = accumulate ? out[t] + v : v;
+  }
+}
+
+// Sub-cell quadratic-fit moments (cell_fit_kernel + cell_moments_kernel<ORDER, 0>): per
+// cell, with A the normal matrix of the 4x4x4 stencil fit and a = sum_s d_s J_s subvol
+// sum_j q_j mono_j(x_s - c) (d_s the fit basis at sample s), the stencil nodes receive
+// d_p . A^-1 a.  Scattered with atomic adds (cells share stencil nodes).
+template <int ORDER>
+__global__ __launch_bounds__(IG_THREADS) void cell_moments_adjoint_kernel(
+    const double* __restrict__ xyz, const double* __restrict__ jac, int nx, int ny, int nz,
+    const int* __restrict__ cells, int ncells, double subvol, double cx, double cy, double cz, MomentWeights q,
+    double* __restrict__ out) {
+  constexpr int NM = nmono(ORDER);
+  const long long N = (long long)nx * ny * nz;
+  for (int c = blockIdx.x * IG_THREADS + threadIdx.x; c < ncells; c += gridDim.x * IG_THREADS) {
+    const int ci = cells[3 * c], cj = cells[3 * c + 1], ck = cells[3 * c + 2];
+    // centroid and scale of the stencil, as the forward fit
+    double ox = 0.0, oy = 0.0, oz = 0.0;
+    for (int a = -1; a <= 2; ++a)
+      for (int b = -1; b <= 2; ++b)
+        for (int d = -1; d <= 2; ++d) {
+          const long long g = ((long long)(ci + a) * ny + (cj + b)) * nz + (ck + d);
+          ox += xyz[g];
+          oy += xyz[N + g];
+          oz += xyz[2 * N + g];
+        }
+    ox *= 1.0 / 64.0;
+    oy *= 1.0 / 64.0;
+    oz *= 1.0 / 64.0;
+    double sc = 0.0;
+    for (int a = -1; a <= 2; ++a)
+      for (int b = -1; b <= 2; ++b)
+        for (int d = -1; d <= 2; ++d) {
+          const long long g = ((long long)(ci + a) * ny + (cj + b)) * nz + (ck + d);
+          sc = fmax(sc, fmax(fabs(xyz[g] - ox), fmax(fabs(xyz[N + g] - oy), fabs(xyz[2 * N + g] - oz))));
+        }
+    const double is = 1.0 / sc;
+    double A[55];
+    for (int e = 0; e < 55; ++e) A[e] = 0.0;
+    for (int a = -1; a <= 2; ++a)
+      for (int b = -1; b <= 2; ++b)
+        for (int d = -1; d <= 2; ++d) {
+          const long long g = ((long long)(ci + a) * ny + (cj + b)) * nz + (ck + d);
+          double m[10];
+          monomials<2>((xyz[g] - ox) * is, (xyz[N + g] - oy) * is, (xyz[2 * N + g] - oz) * is, m);
+          int e = 0;
+          for (int r = 0; r < 10; ++r)
+            for (int t2 = 0; t2 <= r; ++t2) A[e++] += m[r] * m[t2];
+        }
+    for (int r = 0; r < 10; ++r) {
+      for (int t2 = 0; t2 <= r; ++t2) {
+        double v = A[r * (r + 1) / 2 + t2];
+        for (int u = 0; u < t2; ++u) v -= A[r * (r + 1) / 2 + u] * A[t2 * (t2 + 1) / 2 + u];
+        if (r == t2) A[r * (r + 1) / 2 + t2] = sqrt(fmax(v, 1e-300));
+        else A[r * (r + 1) / 2 + t2] = v / A[t2 * (t2 + 1) / 2 + t2];
+      }
+    }
+    // a = sum over the 2x2x2 sub-cell samples of d_s * J_s subvol * (q . mono(x_s - c))
+    double X[8], Y[8], Z[8], Jc[8];
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b)
+        for (int d = 0; d < 2; ++d) {
+          const long long g = ((long long)(ci + a) * ny + (cj + b)) * nz + (ck + d);
+          const int e = a * 4 + b * 2 + d;
+          X[e] = xyz[g];
+          Y[e] = xyz[N + g];
+          Z[e] = xyz[2 * N + g];
+          Jc[e] = jac[g];
+        }
+    double av[10];
+    for (int r = 0; r < 10; ++r) av[r] = 0.0;
+    for (int smp = 0; smp < 8; ++smp) {
+      const double u = (smp & 4) ? 0.75 : 0.25, v = (smp & 2) ? 0.75 : 0.25, ww = (smp & 1) ? 0.75 : 0.25;
+      double xs = 0.0, ys = 0.0, zs = 0.0, js = 0.0;
+      for (int e = 0; e < 8; ++e) {
+        const double fu = (e & 4) ? u : 1.0 - u, fv = (e & 2) ? v : 1.0 - v, fw = (e & 1) ? ww : 1.0 - ww;
+        const double tri = fu * fv * fw;
+        xs += tri * X[e];
+        ys += tri * Y[e];
+        zs += tri * Z[e];
+        js += tri * Jc[e];
+      }
+      double mm[NM];
+      monomials<ORDER>(xs - cx, ys - cy, zs - cz, mm);
+      double qs = 0.0;
+#pragma unroll
+      for (int i = 0; i < NM; ++i) qs += q.q[i] * mm[i];
+      double m2[10];
+      monomials<2>((xs - ox) * is, (ys - oy) * is, (zs - oz) * is, m2);
+      const double wgt = js * subvol * qs;
+      for (int r = 0; r < 10; ++r) av[r] += wgt * m2[r];
+    }
+    // z = A^-1 a through the Cholesky factor
+    double z[10];
+    for (int r = 0; r < 10; ++r) {
+      double v = av[r];
+      for (int t2 = 0; t2 < r; ++t2) v -= A[r * (r + 1) / 2 + t2] * z[t2];
+      z[r] = v / A[r * (r + 1) / 2 + r];
+    }
+    for (int r = 9; r >= 0; --r) {
+      double v = z[r];
+      for (int t2 = r + 1; t2 < 10; ++t2) v -= A[t2 * (t2 + 1) / 2 + r] * z[t2];
+      z[r] = v / A[r * (r + 1) / 2 + r];
+    }
+    for (int a = -1; a <= 2; ++a)
+      for (int b = -1; b <= 2; ++b)
+        for (int d = -1; d <= 2; ++d) {
+          const long long g = ((long long)(ci + a) * ny + (cj + b)) * nz + (ck + d);
+          double m[10];
+          monomials<2>((xyz[g] - ox) * is, (xyz[N + g] - oy) * is, (xyz[2 * N + g] - oz) * is, m);
+          double beta = 0.0;
+          for (int r = 0; r < 10; ++r) beta += m[r] * z[r];
+          atomicAdd(&out[g], beta);
+        }
+  }
+}
+
+template <int ORDER>
+int moments_adjoint_t(const double* w, const double* frac, const double* xyz, long long N, const double* center,
+                      const double* q, double* out, int accumulate, hipStream_t s) {
+  MomentWeights mq;
+  for (int i = 0; i < 35; ++i) mq.q[i] = i < nmono(ORDER) ? q[i] : 0.0;
+  hipLaunchKernelGGL(moments_adjoint_kernel<ORDER>, dim3(grid1d(N)), dim3(IG_THREADS), 0, s, w, frac, xyz, N,
+                     center[0], center[1], center[2], mq, out, accumulate);
+  IG_CHECK_LAUNCH();
+  return IG_OK;
+}
+
+template <int ORDER>
+int cell_moments_adjoint_t(const double* xyz, const double* jac, int nx, int ny, int nz, const int* cells,
+                           int ncells, double subvol, const double* center, const double* q, double* out,
+                           hipStream_t s) {
+  MomentWeights mq;
+  for (int i = 0; i < 35; ++i) mq.q[i] = i < nmono(ORDER) ? q[i] : 0.0;
+  int nb = (ncells + IG_THREADS - 1) / IG_THREADS;
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(cell_moments_adjoint_kernel<ORDER>, dim3(nb), dim3(IG_THREADS), 0, s, xyz, jac, nx, ny, nz,
+                     cells, ncells, subvol, center[0], center[1], center[2], mq, out);
+  IG_CHECK_LAUNCH();
+  return IG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -781,6 +947,35 @@ int ig_cell_split_moments_f64(const double* rho, const double* xyz, const double
     case 2: return cell_moments_t<2, 2>(xyz, jac, nx, ny, nz, cells, ncells, coef, subvol, center, work, out, s, rho);
     case 3: return cell_moments_t<3, 2>(xyz, jac, nx, ny, nz, cells, ncells, coef, subvol, center, work, out, s, rho);
     case 4: return cell_moments_t<4, 2>(xyz, jac, nx, ny, nz, cells, ncells, coef, subvol, center, work, out, s, rho);
+    default: return IG_EUNSUPPORTED;
+  }
+}
+
+int ig_moments_adjoint_f64(const double* w, const double* frac, const double* xyz, long long N,
+                           const double* center, int order, const double* q, double* out, int accumulate,
+                           hipStream_t s) {
+  if (!w || !xyz || !center || !q || !out || N <= 0) return IG_EINVAL;
+  switch (order) {
+    case 0: return moments_adjoint_t<0>(w, frac, xyz, N, center, q, out, accumulate, s);
+    case 1: return moments_adjoint_t<1>(w, frac, xyz, N, center, q, out, accumulate, s);
+    case 2: return moments_adjoint_t<2>(w, frac, xyz, N, center, q, out, accumulate, s);
+    case 3: return moments_adjoint_t<3>(w, frac, xyz, N, center, q, out, accumulate, s);
+    case 4: return moments_adjoint_t<4>(w, frac, xyz, N, center, q, out, accumulate, s);
+    default: return IG_EUNSUPPORTED;
+  }
+}
+
+int ig_cell_moments_adjoint_f64(const double* xyz, const double* jac, int nx, int ny, int nz, const int* cells,
+                                int ncells, double subvol, const double* center, int order, const double* q,
+                                double* out, hipStream_t s) {
+  if (!xyz || !jac || !cells || !center || !q || !out || ncells < 0) return IG_EINVAL;
+  if (ncells == 0) return IG_OK;
+  switch (order) {
+    case 0: return cell_moments_adjoint_t<0>(xyz, jac, nx, ny, nz, cells, ncells, subvol, center, q, out, s);
+    case 1: return cell_moments_adjoint_t<1>(xyz, jac, nx, ny, nz, cells, ncells, subvol, center, q, out, s);
+    case 2: return cell_moments_adjoint_t<2>(xyz, jac, nx, ny, nz, cells, ncells, subvol, center, q, out, s);
+    case 3: return cell_moments_adjoint_t<3>(xyz, jac, nx, ny, nz, cells, ncells, subvol, center, q, out, s);
+    case 4: return cell_moments_adjoint_t<4>(xyz, jac, nx, ny, nz, cells, ncells, subvol, center, q, out, s);
     default: return IG_EUNSUPPORTED;
   }
 }

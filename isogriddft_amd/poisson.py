@@ -218,6 +218,14 @@ def quadrupole_from_flat(flat) -> np.ndarray:
 class PoissonGridData:
     """Geometry-only data of the open-boundary solve, built once per grid."""
 
+    def interior_mask(self, dtype):
+        """1 on interior nodes, 0 on the faces, in the given dtype (flat)."""
+        m = self._masks.get(dtype)
+        if m is None:
+            m = self.interior.to(dtype)
+            self._masks[dtype] = m
+        return m
+
     def __init__(self, dg: DeviceGrid):
         self.dg = dg
         nx, ny, nz = dg.shape
@@ -225,6 +233,7 @@ class PoissonGridData:
         interior = np.zeros(dg.shape, dtype=bool)
         interior[1:-1, 1:-1, 1:-1] = True
         self.interior = torch.from_numpy(interior.reshape(-1)).to(dev)
+        self._masks = {}
         self.n_interior = int(interior.sum())
         self.dinv = None
         self.cells = None
@@ -420,14 +429,21 @@ def _space(n, dtype, device):
     return sp
 
 
-def _matvec(dg, x, out, stream=None):
-    """out = -L x on interior nodes, 0 on the faces (production closure on monitor grids)."""
-    apply_operator(dg, x.view(dg.shape), out.view(dg.shape), scale=-1.0, closure="production",
-                   interior_only=True, stream=stream)
+def _matvec(dg, x, out, stream=None, transpose=False):
+    """out = -L x on interior nodes, 0 on the faces (production closure on monitor grids);
+    with transpose, the transposed interior operator S (-L^T) E for x with zero faces."""
+    if not transpose:
+        apply_operator(dg, x.view(dg.shape), out.view(dg.shape), scale=-1.0, closure="production",
+                       interior_only=True, stream=stream)
+        return
+    apply_operator(dg, x.view(dg.shape), out.view(dg.shape), scale=-1.0, closure="production", transpose=True,
+                   stream=stream)
+    out.mul_(poisson_data(dg).interior_mask(out.dtype))
 
 
-def bicgstab(dg, b, x, *, dinv=None, rtol=1e-8, maxiter=400, check_every=8, stream=None):
-    """Solve (-L) x = b on interior nodes; x (flat CUDA tensor) is overwritten.
+def bicgstab(dg, b, x, *, dinv=None, rtol=1e-8, maxiter=400, check_every=8, stream=None, transpose=False):
+    """Solve (-L) x = b on interior nodes (transpose: (-L)^T x = b); x (flat CUDA tensor)
+    is overwritten.
 
     b and x share a dtype (float32 / float64); Krylov scalars stay FP64 on the device.
     Returns (iterations, converged, relative residual ||r|| / ||b||).
@@ -461,11 +477,11 @@ def bicgstab(dg, b, x, *, dinv=None, rtol=1e-8, maxiter=400, check_every=8, stre
         it += 1
         _lib.call(f"ig_bicgstab_p_{sfx}", n, _lib.ptr(sp.r), _lib.ptr(sp.p), _lib.ptr(sp.v), dv,
                   _lib.ptr(sp.phat), _lib.ptr(sc), h)
-        _matvec(dg, sp.phat, sp.v, stream)
+        _matvec(dg, sp.phat, sp.v, stream, transpose)
         dots_into(sc, [(sp.rhat, sp.v)], None, slots=(4, 0, 0), stream=stream)
         _lib.call(f"ig_bicgstab_s_{sfx}", n, _lib.ptr(sp.r), _lib.ptr(sp.v), dv, _lib.ptr(sp.s),
                   _lib.ptr(sp.shat), _lib.ptr(sc), h)
-        _matvec(dg, sp.shat, sp.t, stream)
+        _matvec(dg, sp.shat, sp.t, stream, transpose)
         dots_into(sc, [(sp.t, sp.s), (sp.t, sp.t)], None, slots=(5, 6, 0), stream=stream)
         _lib.call(f"ig_bicgstab_x_{sfx}", n, _lib.ptr(x), _lib.ptr(sp.phat), _lib.ptr(sp.shat), _lib.ptr(sp.s),
                   _lib.ptr(sp.t), _lib.ptr(sp.r), _lib.ptr(sp.rhat), _lib.ptr(sc), _lib.ptr(sp.work), h)
@@ -596,6 +612,73 @@ def solve_poisson_device(dg, rho, *, multipole_order=2, tolerance=1e-8, max_iter
         "residual_max": residual_max,
         "method": method,
     }
+
+
+# ------------------------------------------------------------------ adjoint
+
+
+def moments_adjoint_device(dg, q, order, mode="corrected_moments", out=None, stream=None):
+    """sum_j q_j d moment_j / d rho (flat FP64 field) for the flat raw moments that
+    multipole_boundary_device reports (nodal; on monitor grids plus the corrected-moment
+    delta, whose sub-cell quadratic-fit part is scattered by ig_cell_moments_adjoint_f64)."""
+    q = np.ascontiguousarray(np.asarray(q, dtype=np.float64))
+    if q.size != nmono(order):
+        raise ValueError(f"q must hold {nmono(order)} moment weights for order {order}.")
+    pdata = poisson_data(dg)
+    if out is None:
+        out = torch.empty(dg.N, dtype=torch.float64, device=dg.device)
+    import ctypes
+
+    qp = q.ctypes.data_as(ctypes.c_void_p)
+    cptr, keep = _center_arg(dg.reference_center)
+    corrected = dg.kind == "monitor" and pdata.ncells > 0
+    _lib.call("ig_moments_adjoint_f64", _lib.ptr(dg.weights()), _lib.ptr(pdata.frac if corrected else None),
+              _lib.ptr(dg.xyz()), dg.N, cptr, order, qp, _lib.ptr(out), 0, _lib.stream_handle(stream))
+    if corrected:
+        nx, ny, nz = dg.shape
+        _lib.call("ig_cell_moments_adjoint_f64", _lib.ptr(dg.xyz()), _lib.ptr(dg.get("jac")), nx, ny, nz,
+                  _lib.ptr(pdata.cells), pdata.ncells, pdata.subvol, cptr, order, qp, _lib.ptr(out),
+                  _lib.stream_handle(stream))
+    del keep
+    return out
+
+
+def poisson_adjoint_device(dg, g, multipole_order=2, mode="corrected_moments", tolerance=1e-10,
+                           max_iterations=4000, stream=None):
+    """P^T g for the open-boundary Poisson map v = P rho of solve_poisson_device: the
+    gradient of <g, v(rho)> with respect to rho.
+
+    With b = B rho the multipole faces, S / E interior restriction / extension and
+    A = S (-L) E, the forward map is v = B rho + E A^-1 S (4 pi rho + L B rho), so
+        P^T g = 4 pi E y + B^T (g + L^T E y),   A^T y = S g,
+    solved by BiCGSTAB on the transposed stencil (no stored forward iterates), and
+    B^T h = M^T F^T h with F the faces of each unit moment and M^T the moment adjoint.
+    The legacy_split construction is not linear in rho (its correction centre is
+    |d|-weighted) and has no adjoint here.  Returns (gradient, info dict)."""
+    if dg.kind == "monitor" and mode == "legacy_split":
+        raise NotImplementedError("the legacy_split boundary is not linear in rho; its adjoint is not built.")
+    if mode not in _MODES:
+        raise ValueError(f"monitor_boundary_construction_mode must be one of {_MODES}; received `{mode}`.")
+    pdata = poisson_data(dg)
+    g = g.reshape(-1).to(torch.float64).contiguous()
+    b = g * pdata.interior_mask(torch.float64)
+    y = torch.empty_like(b)
+    it, conv, rel = bicgstab(dg, b, y, dinv=pdata.dinv, rtol=tolerance, maxiter=max_iterations, stream=stream,
+                             transpose=True)
+    lty = apply_operator(dg, y.view(dg.shape), scale=1.0, closure="production", transpose=True,
+                         stream=stream).reshape(-1)
+    h = g + lty
+    from .reductions import wdot
+
+    nm = nmono(multipole_order)
+    q = np.empty(nm)
+    for j in range(nm):
+        e = np.zeros(nm)
+        e[j] = 1.0
+        q[j] = wdot(dg, boundary_field(dg, e, multipole_order, None, stream), h, weighted=False)
+    grad = moments_adjoint_device(dg, q, multipole_order, mode, stream=stream)
+    grad.add_(y, alpha=FOUR_PI)
+    return grad, {"iterations": int(it), "converged": bool(conv), "residual": float(rel)}
 
 
 # ------------------------------------------------------------------ drop-in call surfaces
