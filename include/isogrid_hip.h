@@ -305,6 +305,23 @@ int ig_mapping_metric_f64(const double* coords, int nx, int ny, int nz, double h
                           double* jacobian, double* inverse_metric, double* cell_volumes,
                           hipStream_t stream);
 
+/* ------------------------------------------------------------------ Green identity */
+
+/* Doubles of `work` needed by ig_green_identity_f64. */
+int ig_green_identity_work_size(void);
+/* Discrete Green-identity terms of a real field f on a monitor grid; out[4] =
+ *   { <f, tf>_W,  1/2 sum w d_a f g^{ab} d_b f,  boundary flux term,  <f, f>_W }.
+ * tf = T f from any closure (NULL: out[0] = 0); coef = c^{ab} = J g^{ab} [6, N] in the
+ * order xx, yy, zz, xy, xz, yz; invj = 1/J; w = cell volumes.  The gradient form uses
+ * np.gradient(edge_order=2) derivatives; the boundary term sums +-1/2 f F^a dS over the
+ * face nodes that lie on no edge, with F^a = c^{ab} d_b f from `closure` (0 zero-ghost
+ * centred, 1 np.gradient edge_order 2).  Deterministic fixed-order reduction.
+ * Replaces: audit/h2_monitor_grid_kinetic_green_identity_audit.py:143-218 (gradient
+ * density, boundary proxy) and :295-330 (operator / gradient / boundary energies). */
+int ig_green_identity_f64(const double* f, const double* tf, const double* coef, const double* invj,
+                          const double* w, int nx, int ny, int nz, double hx, double hy, double hz, int closure,
+                          double* work, double* out, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

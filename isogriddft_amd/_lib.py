@@ -69,6 +69,8 @@ SIGNATURES = {
     "ig_harmonic_sweep_f64": [P, P, P, I, I, I, D, D, D, D, P, P, P],
     "ig_mapping_min_jacobian_f64": [P, I, I, I, D, D, D, P, P, P],
     "ig_mapping_metric_f64": [P, I, I, I, D, D, D, P, P, P, P],
+    "ig_green_identity_work_size": [],
+    "ig_green_identity_f64": [P, P, P, P, P, I, I, I, D, D, D, I, P, P, P],
 }
 _RESTYPE = {"ig_gram_work_size": L, "ig_knn_work_bytes": L}
 

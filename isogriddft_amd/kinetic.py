@@ -6,12 +6,16 @@ Call surfaces (same names and argument meaning as the reference):
   apply_monitor_grid_kinetic_operator[_trial_boundary_fix]             ops/kinetic.py:340-358
   apply_laplacian_operator / apply_kinetic_operator                    ops/kinetic.py:361-374
   integrate_field / weighted_l2_norm                                   ops/kinetic.py:61-74
+  green_identity_terms        audit/h2_monitor_grid_kinetic_green_identity_audit.py:143-218, :295-330
+  green_identity_defect       <u, T v>_W - <T u, v>_W (the operator/adjoint mismatch)
 NumPy in -> NumPy out (FP64), as the reference; a CUDA tensor in -> a CUDA tensor
 out in the tensor's dtype (float32 or float64), which is the device-resident path
 used by the solvers.  ``apply_operator`` is the block-level entry point.
 """
 
 from __future__ import annotations
+
+from dataclasses import dataclass
 
 import numpy as np
 import torch
@@ -148,3 +152,80 @@ def weighted_l2_norm(field, grid_geometry):
     from .reductions import wdot
     v = t.reshape(-1)
     return float(np.sqrt(max(wdot(dg, v, v), 0.0)))
+
+
+# ---------------------------------------------------------------- Green identity
+
+
+@dataclass(frozen=True)
+class GreenIdentityTerms:
+    """Discrete Green-identity energies of one real field, all normalised by <f, f>_W
+    (the scalar fields of audit/h2_monitor_grid_kinetic_green_identity_audit.py:88-111)."""
+
+    closure: str
+    weighted_norm: float
+    operator_kinetic_ha: float
+    gradient_kinetic_ha: float
+    delta_kinetic_mha: float
+    boundary_term_ha: float
+    closure_mismatch_mha: float
+
+
+def _field_tensor(field, dg):
+    t = field if isinstance(field, torch.Tensor) else torch.from_numpy(
+        np.ascontiguousarray(field, dtype=np.float64)).to(dg.device)
+    if tuple(t.shape) != dg.shape:
+        raise ValueError(f"field must match the grid shape {dg.shape}; received {tuple(t.shape)}.")
+    if t.dtype != torch.float64:
+        t = t.to(torch.float64)
+    return t.contiguous()
+
+
+def green_identity_terms(field, grid_geometry, closure: str = "trial_fix", *, stream=None) -> GreenIdentityTerms:
+    """Operator energy <f, T f>_W with T of ``closure``, the gradient-form energy
+    1/2 sum w d_a f g^{ab} d_b f (np.gradient edge_order=2), the explicit boundary-flux
+    term with F^a = J g^{ab} d_b f of ``closure``, and the closure mismatch
+    K_op - K_grad - K_bdry, each divided by <f, f>_W
+    (audit/h2_monitor_grid_kinetic_green_identity_audit.py:143-218, :295-330; the
+    production-closure numbers follow the earlier audit recorded in
+    audit/baselines.py:1473-1545).  Everything runs on the device in FP64."""
+    dg = device_grid(grid_geometry)
+    if dg.kind != "monitor":
+        raise ValueError("the Green-identity terms are defined on monitor grids")
+    cl = _closure_id(closure)
+    f = _field_tensor(field, dg)
+    tf = apply_operator(dg, f, scale=-0.5, closure=closure, stream=stream)
+    from .reductions import workspace
+    work = workspace(dg.device, int(_lib.load().ig_green_identity_work_size()), slot="green")
+    out = torch.empty(4, dtype=torch.float64, device=dg.device)
+    nx, ny, nz = dg.shape
+    hx, hy, hz = dg.h
+    _lib.call("ig_green_identity_f64", _lib.ptr(f), _lib.ptr(tf), _lib.ptr(dg.coef()), _lib.ptr(dg.invj()),
+              _lib.ptr(dg.weights()), nx, ny, nz, float(hx), float(hy), float(hz), cl, _lib.ptr(work),
+              _lib.ptr(out), _lib.stream_handle(stream))
+    k_op, k_grad, k_bdry, den = (float(v) for v in out.cpu())
+    if not den > 0.0:
+        raise ValueError("field has zero weighted norm")
+    return GreenIdentityTerms(
+        closure=closure,
+        weighted_norm=float(np.sqrt(den)),
+        operator_kinetic_ha=k_op / den,
+        gradient_kinetic_ha=k_grad / den,
+        delta_kinetic_mha=1000.0 * (k_op - k_grad) / den,
+        boundary_term_ha=k_bdry / den,
+        closure_mismatch_mha=1000.0 * (k_op - k_grad - k_bdry) / den,
+    )
+
+
+def green_identity_defect(u, v, grid_geometry, closure: str = "production", *, stream=None) -> float:
+    """<u, T v>_W - <T u, v>_W: the mismatch between the kinetic operator and its adjoint
+    under the quadrature weights (zero to round-off for the zero-ghost closure, whose
+    transpose is W T W^-1; nonzero for the one-sided production ends).  Legacy grids use
+    the product weights under which the 7-point operator is self-adjoint."""
+    dg = device_grid(grid_geometry)
+    a = _field_tensor(u, dg)
+    b = _field_tensor(v, dg)
+    ta = apply_operator(dg, a, scale=-0.5, closure=closure, stream=stream)
+    tb = apply_operator(dg, b, scale=-0.5, closure=closure, stream=stream)
+    w = dg.operator_weights()
+    return float(torch.sum(a.reshape(-1) * tb.reshape(-1) * w) - torch.sum(ta.reshape(-1) * b.reshape(-1) * w))

@@ -243,3 +243,41 @@ def legacy_laplacian_matrix(geom):
         + sp.kron(sp.kron(Ix, mats[1]), Iz)
         + sp.kron(sp.kron(Ix, Iy), mats[2])
     ).tocsr()
+
+
+# ---------------------------------------------------------------- Green identity
+
+
+def green_identity_terms(f, geom, closure="trial_fix"):
+    """(K_op, K_grad, K_bdry, <f,f>_W), unnormalised, for a real field f on a monitor grid
+    (audit/h2_monitor_grid_kinetic_green_identity_audit.py:143-218, :295-330):
+      K_op   = sum w f T f                      T of ``closure``
+      K_grad = 1/2 sum w d_a f g^{ab} d_b f     np.gradient edge_order=2 (:156-176)
+      K_bdry = sum_faces s 1/2 f F^a dS         face nodes off every edge, s = +1 at the
+               minimum face, -1 at the maximum face, F^a = J g^{ab} d_b f of ``closure``
+               (:179-218)."""
+    hs = (_spacing(geom.logical_x), _spacing(geom.logical_y), _spacing(geom.logical_z))
+    w = np.asarray(geom.cell_volumes, dtype=np.float64)
+    J = np.asarray(geom.jacobian, dtype=np.float64)
+    gi = np.asarray(geom.inverse_metric_tensor, dtype=np.float64)
+    f = np.asarray(f, dtype=np.float64)
+    k_op = float(np.sum(w * f * monitor_kinetic(f, geom, closure)))
+    gp = [_grad_production(f, hs[a], a) for a in range(3)]
+    dens = np.zeros_like(f)
+    for a in range(3):
+        for b in range(3):
+            dens += gp[a] * gi[..., a, b] * gp[b]
+    k_grad = float(np.sum(0.5 * dens * w))
+    gc = [_grad(f, hs[a], a, closure) for a in range(3)]
+    flux = [J * (gi[..., a, 0] * gc[0] + gi[..., a, 1] * gc[1] + gi[..., a, 2] * gc[2]) for a in range(3)]
+    inner = slice(1, -1)
+    k_bdry = 0.0
+    for a in range(3):
+        area = hs[(a + 1) % 3] * hs[(a + 2) % 3]
+        for idx, sgn in ((0, 0.5), (-1, -0.5)):
+            sl = [inner, inner, inner]
+            sl[a] = idx
+            sl = tuple(sl)
+            k_bdry += float(np.sum(sgn * f[sl] * flux[a][sl]) * area)
+    den = float(np.sum(w * f * f))
+    return k_op, k_grad, k_bdry, den
