@@ -305,6 +305,32 @@ int ig_mapping_metric_f64(const double* coords, int nx, int ny, int nz, double h
                           double* jacobian, double* inverse_metric, double* cell_volumes,
                           hipStream_t stream);
 
+/* ------------------------------------------------------------------ row-wise block CG */
+
+/* Doubles of `work` needed by the three row-wise entries below for k rows. */
+int ig_row_dots_work_size(int k);
+/* out[i] = sum_n w[n] a[i][n] b[i][n] (w NULL: 1) for i < k, rows of length N at stride ld;
+ * FP64 accumulation, deterministic.  No reference counterpart: the reference's block
+ * iteration is unpreconditioned; these carry the kinetic preconditioner added here
+ * (k conjugate-gradient solves of (T + c) x = r, one per residual row). */
+int ig_row_dots_f64(const double* a, const double* b, const double* w, int k, long long N, long long ld,
+                    double* work, double* out, hipStream_t stream);
+int ig_row_dots_f32(const float* a, const float* b, const float* w, int k, long long N, long long ld,
+                    double* work, double* out, hipStream_t stream);
+/* Row i: x += alpha[i] p, r -= alpha[i] q, out[i] = sum_n w[n] dinv[n] r[i][n]^2 (alpha and
+ * out FP64 device vectors of length k).  No reference counterpart (see above). */
+int ig_cg_update_f64(double* x, double* r, const double* p, const double* q, const double* dinv, const double* w,
+                     const double* alpha, int k, long long N, long long ld, double* work, double* out,
+                     hipStream_t stream);
+int ig_cg_update_f32(float* x, float* r, const float* p, const float* q, const float* dinv, const float* w,
+                     const double* alpha, int k, long long N, long long ld, double* work, double* out,
+                     hipStream_t stream);
+/* Row i: p = dinv r + beta[i] p.  No reference counterpart (see above). */
+int ig_cg_direction_f64(double* p, const double* r, const double* dinv, const double* beta, int k, long long N,
+                        long long ld, hipStream_t stream);
+int ig_cg_direction_f32(float* p, const float* r, const float* dinv, const double* beta, int k, long long N,
+                        long long ld, hipStream_t stream);
+
 /* ------------------------------------------------------------------ Green identity */
 
 /* Doubles of `work` needed by ig_green_identity_f64. */

@@ -69,6 +69,13 @@ SIGNATURES = {
     "ig_harmonic_sweep_f64": [P, P, P, I, I, I, D, D, D, D, P, P, P],
     "ig_mapping_min_jacobian_f64": [P, I, I, I, D, D, D, P, P, P],
     "ig_mapping_metric_f64": [P, I, I, I, D, D, D, P, P, P, P],
+    "ig_row_dots_work_size": [I],
+    "ig_row_dots_f64": [P, P, P, I, L, L, P, P, P],
+    "ig_row_dots_f32": [P, P, P, I, L, L, P, P, P],
+    "ig_cg_update_f64": [P, P, P, P, P, P, P, I, L, L, P, P, P],
+    "ig_cg_update_f32": [P, P, P, P, P, P, P, I, L, L, P, P, P],
+    "ig_cg_direction_f64": [P, P, P, P, I, L, L, P],
+    "ig_cg_direction_f32": [P, P, P, P, I, L, L, P],
     "ig_green_identity_work_size": [],
     "ig_green_identity_f64": [P, P, P, P, P, I, I, I, D, D, D, I, P, P, P],
 }

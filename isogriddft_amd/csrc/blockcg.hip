@@ -1,0 +1,174 @@
+// Row-wise block conjugate-gradient kernels: k independent CG iterations, one per row
+// of a (k, N) block with row stride ld, under the inner product <a, b>_w = sum w a b.
+// They carry the kinetic preconditioner of the block eigensolver (a few CG steps on
+// (T + c) x = r per residual row); the operator apply itself is the stencil kernel.
+//
+//   row_dots:     out[i] = sum_n w[n] a[i][n] b[i][n]
+//   cg_update:    x += alpha_i p, r -= alpha_i q, out[i] = sum_n w[n] dinv[n] r[i][n]^2
+//   cg_direction: p = dinv r + beta_i p
+// alpha / beta are FP64 device vectors of length k (no host round trip).  Sums are FP64;
+// each (row, chunk) block writes a partial and one block per row adds the partials in a
+// fixed order, so results are bit-reproducible.
+#include "common.h"
+
+namespace {
+
+constexpr int CG_MAX_CHUNKS = 256;
+
+template <typename T>
+__global__ __launch_bounds__(IG_THREADS) void row_dots_kernel(const T* __restrict__ a, const T* __restrict__ b,
+                                                              const T* __restrict__ w, long long N, long long ld,
+                                                              long long chunk, double* __restrict__ partial) {
+  __shared__ double sm[4];
+  const int row = blockIdx.y;
+  const long long beg = (long long)blockIdx.x * chunk;
+  const long long end = min(beg + chunk, N);
+  const T* ar = a + row * ld;
+  const T* br = b + row * ld;
+  double s = 0.0;
+  for (long long n = beg + threadIdx.x; n < end; n += IG_THREADS) {
+    const double v = (double)ar[n] * (double)br[n];
+    s += w ? v * (double)w[n] : v;
+  }
+  double v[1] = {s};
+  ig_block_sum<1>(v, sm);
+  if (threadIdx.x == 0) partial[(long long)row * gridDim.x + blockIdx.x] = v[0];
+}
+
+template <typename T>
+__global__ __launch_bounds__(IG_THREADS) void cg_update_kernel(T* __restrict__ x, T* __restrict__ r,
+                                                               const T* __restrict__ p, const T* __restrict__ q,
+                                                               const T* __restrict__ dinv, const T* __restrict__ w,
+                                                               const double* __restrict__ alpha, long long N,
+                                                               long long ld, long long chunk,
+                                                               double* __restrict__ partial) {
+  __shared__ double sm[4];
+  const int row = blockIdx.y;
+  const long long beg = (long long)blockIdx.x * chunk;
+  const long long end = min(beg + chunk, N);
+  const T a = (T)alpha[row];
+  T* xr = x + row * ld;
+  T* rr = r + row * ld;
+  const T* pr = p + row * ld;
+  const T* qr = q + row * ld;
+  double s = 0.0;
+  for (long long n = beg + threadIdx.x; n < end; n += IG_THREADS) {
+    xr[n] = xr[n] + a * pr[n];
+    const T rn = rr[n] - a * qr[n];
+    rr[n] = rn;
+    const double v = (double)rn * (double)rn * (double)dinv[n];
+    s += w ? v * (double)w[n] : v;
+  }
+  double v[1] = {s};
+  ig_block_sum<1>(v, sm);
+  if (threadIdx.x == 0) partial[(long long)row * gridDim.x + blockIdx.x] = v[0];
+}
+
+template <typename T>
+__global__ __launch_bounds__(IG_THREADS) void cg_direction_kernel(T* __restrict__ p, const T* __restrict__ r,
+                                                                  const T* __restrict__ dinv,
+                                                                  const double* __restrict__ beta, long long N,
+                                                                  long long ld, long long chunk) {
+  const int row = blockIdx.y;
+  const long long beg = (long long)blockIdx.x * chunk;
+  const long long end = min(beg + chunk, N);
+  const T b = (T)beta[row];
+  T* pr = p + row * ld;
+  const T* rr = r + row * ld;
+  for (long long n = beg + threadIdx.x; n < end; n += IG_THREADS) pr[n] = dinv[n] * rr[n] + b * pr[n];
+}
+
+__global__ void rows_finish(const double* __restrict__ partial, int nparts, int k, double* __restrict__ out) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int c = 0; c < nparts; ++c) s += partial[(long long)i * nparts + c];
+    out[i] = s;
+  }
+}
+
+struct RowPlan {
+  int nch;
+  long long chunk;
+};
+
+RowPlan row_plan(int k, long long N) {
+  long long nch = (2048 + k - 1) / k;
+  const long long cap = (N + IG_THREADS - 1) / IG_THREADS;  // at least one element per thread
+  if (nch > cap) nch = cap;
+  if (nch > CG_MAX_CHUNKS) nch = CG_MAX_CHUNKS;
+  if (nch < 1) nch = 1;
+  RowPlan g;
+  g.chunk = (N + nch - 1) / nch;
+  g.nch = (int)((N + g.chunk - 1) / g.chunk);
+  return g;
+}
+
+template <typename T>
+int row_dots(const T* a, const T* b, const T* w, int k, long long N, long long ld, double* work, double* out,
+             hipStream_t s) {
+  if (!a || !b || !work || !out || k <= 0 || k > 65535 || N <= 0 || ld < N) return IG_EINVAL;
+  const RowPlan g = row_plan(k, N);
+  hipLaunchKernelGGL(row_dots_kernel<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, a, b, w, N, ld, g.chunk, work);
+  hipLaunchKernelGGL(rows_finish, dim3((k + 255) / 256), dim3(256), 0, s, work, g.nch, k, out);
+  IG_CHECK_LAUNCH();
+  return IG_OK;
+}
+
+template <typename T>
+int cg_update(T* x, T* r, const T* p, const T* q, const T* dinv, const T* w, const double* alpha, int k, long long N,
+              long long ld, double* work, double* out, hipStream_t s) {
+  if (!x || !r || !p || !q || !dinv || !alpha || !work || !out || k <= 0 || k > 65535 || N <= 0 || ld < N)
+    return IG_EINVAL;
+  const RowPlan g = row_plan(k, N);
+  hipLaunchKernelGGL(cg_update_kernel<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, x, r, p, q, dinv, w, alpha, N, ld,
+                     g.chunk, work);
+  hipLaunchKernelGGL(rows_finish, dim3((k + 255) / 256), dim3(256), 0, s, work, g.nch, k, out);
+  IG_CHECK_LAUNCH();
+  return IG_OK;
+}
+
+template <typename T>
+int cg_direction(T* p, const T* r, const T* dinv, const double* beta, int k, long long N, long long ld,
+                 hipStream_t s) {
+  if (!p || !r || !dinv || !beta || k <= 0 || k > 65535 || N <= 0 || ld < N) return IG_EINVAL;
+  const RowPlan g = row_plan(k, N);
+  hipLaunchKernelGGL(cg_direction_kernel<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, p, r, dinv, beta, N, ld,
+                     g.chunk);
+  IG_CHECK_LAUNCH();
+  return IG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ig_row_dots_work_size(int k) { return k * CG_MAX_CHUNKS; }
+
+int ig_row_dots_f64(const double* a, const double* b, const double* w, int k, long long N, long long ld,
+                    double* work, double* out, hipStream_t s) {
+  return row_dots<double>(a, b, w, k, N, ld, work, out, s);
+}
+int ig_row_dots_f32(const float* a, const float* b, const float* w, int k, long long N, long long ld,
+                    double* work, double* out, hipStream_t s) {
+  return row_dots<float>(a, b, w, k, N, ld, work, out, s);
+}
+int ig_cg_update_f64(double* x, double* r, const double* p, const double* q, const double* dinv, const double* w,
+                     const double* alpha, int k, long long N, long long ld, double* work, double* out,
+                     hipStream_t s) {
+  return cg_update<double>(x, r, p, q, dinv, w, alpha, k, N, ld, work, out, s);
+}
+int ig_cg_update_f32(float* x, float* r, const float* p, const float* q, const float* dinv, const float* w,
+                     const double* alpha, int k, long long N, long long ld, double* work, double* out,
+                     hipStream_t s) {
+  return cg_update<float>(x, r, p, q, dinv, w, alpha, k, N, ld, work, out, s);
+}
+int ig_cg_direction_f64(double* p, const double* r, const double* dinv, const double* beta, int k, long long N,
+                        long long ld, hipStream_t s) {
+  return cg_direction<double>(p, r, dinv, beta, k, N, ld, s);
+}
+int ig_cg_direction_f32(float* p, const float* r, const float* dinv, const double* beta, int k, long long N,
+                        long long ld, hipStream_t s) {
+  return cg_direction<float>(p, r, dinv, beta, k, N, ld, s);
+}
+
+}  // extern "C"

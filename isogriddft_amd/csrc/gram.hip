@@ -161,12 +161,13 @@ inline unsigned reduce_blocks(long long mn) { return (unsigned)((mn + GR_OUT - 1
 // ---------------------------------------------------------------------------------
 // Narrow products (m <= 256 rows of X, n <= 64 rows of Y): the whole m x n output fits
 // one workgroup, so the grid splits only K.  Thread (tm, tn) = (tid / 8, tid % 8) owns
-// rows tm + 32 a (a < 8) and columns tn + 8 b (b < RN) in registers; a K tile of 16 is
+// rows tm + 32 a (a < RM) and columns tn + 8 b (b < RN) in registers (RM, RN the
+// smallest powers of two covering m / 32 and n / 8, so no FMA is spent on rows or
+// columns outside the product); a K tile of 16 is
 // staged through LDS (K-major, FP64) with the next tile's loads in flight in registers.
 // Each pass over the operands is HBM-bound; the FP64 partials per K chunk are summed by
 // gram_reduce in a fixed order (deterministic).
 constexpr int NW_KT = 16, NW_RM = 8, NW_MMAX = 32 * NW_RM, NW_NMAX = 64;
-constexpr int NW_XP = NW_MMAX * NW_KT / 2 / IG_THREADS;  // X pairs per thread and tile (max)
 constexpr int NW_YP = NW_NMAX * NW_KT / 2 / IG_THREADS;  // Y pairs per thread and tile (max)
 
 template <typename T>
@@ -180,21 +181,22 @@ struct Pair<double> {
   typedef double t __attribute__((ext_vector_type(2)));
 };
 
-template <typename T, int RN, bool PAIR>
+template <typename T, int RM, int RN, bool PAIR>
 __global__ __launch_bounds__(IG_THREADS) void gram_narrow(const T* __restrict__ X, long long ldx, int m,
                                                           const T* __restrict__ Y, long long ldy, int n,
                                                           const T* __restrict__ w, long long N, long long kchunk,
                                                           double* __restrict__ work) {
   using PT = typename Pair<T>::t;
+  constexpr int NW_XP = 32 * RM * NW_KT / 2 / IG_THREADS;  // X pairs per thread and tile
   __shared__ double sx[NW_KT][NW_MMAX];
   __shared__ double sy[NW_KT][NW_NMAX];
   const int tid = threadIdx.x;
   const int tm = tid >> 3, tn = tid & 7;
   const long long kbeg = (long long)blockIdx.x * kchunk;
   const long long kend = min(kbeg + kchunk, N);
-  double acc[NW_RM][RN];
+  double acc[RM][RN];
 #pragma unroll
-  for (int a = 0; a < NW_RM; ++a)
+  for (int a = 0; a < RM; ++a)
 #pragma unroll
     for (int b = 0; b < RN; ++b) acc[a][b] = 0.0;
   // staging (PAIR): element pair q of a tile is (row q / 8, k = 2 (q % 8)); pairs never
@@ -297,20 +299,20 @@ __global__ __launch_bounds__(IG_THREADS) void gram_narrow(const T* __restrict__ 
     if (k0 + NW_KT < kend) fetch(k0 + NW_KT);
 #pragma unroll 4
     for (int kk = 0; kk < NW_KT; ++kk) {
-      double xa[NW_RM], yb[RN];
+      double xa[RM], yb[RN];
 #pragma unroll
-      for (int a = 0; a < NW_RM; ++a) xa[a] = sx[kk][tm + 32 * a];
+      for (int a = 0; a < RM; ++a) xa[a] = sx[kk][tm + 32 * a];
 #pragma unroll
       for (int b = 0; b < RN; ++b) yb[b] = sy[kk][tn + 8 * b];
 #pragma unroll
-      for (int a = 0; a < NW_RM; ++a)
+      for (int a = 0; a < RM; ++a)
 #pragma unroll
         for (int b = 0; b < RN; ++b) acc[a][b] = fma(xa[a], yb[b], acc[a][b]);
     }
   }
   double* out = work + (long long)blockIdx.x * m * n;
 #pragma unroll
-  for (int a = 0; a < NW_RM; ++a)
+  for (int a = 0; a < RM; ++a)
 #pragma unroll
     for (int b = 0; b < RN; ++b) {
       const int i = tm + 32 * a, j = tn + 8 * b;
@@ -353,23 +355,35 @@ int gram_narrow_launch(const NarrowPlan& g, const T* X, long long ldx, int m, co
   const long long lda = g.swap ? ldy : ldx, ldb = g.swap ? ldx : ldy;
   const int ma = g.swap ? n : m, nb = g.swap ? m : n;
   const dim3 grid(g.nchunk), block(IG_THREADS);
-#define IG_NARROW(RN_)                                                                                          \
-  do {                                                                                                          \
-    if (g.pair)                                                                                                 \
-      hipLaunchKernelGGL((gram_narrow<T, RN_, true>), grid, block, 0, s, A, lda, ma, B, ldb, nb, w, N, g.kchunk, \
-                         work);                                                                                 \
-    else                                                                                                        \
-      hipLaunchKernelGGL((gram_narrow<T, RN_, false>), grid, block, 0, s, A, lda, ma, B, ldb, nb, w, N,          \
-                         g.kchunk, work);                                                                       \
+#define IG_NARROW(RM_, RN_)                                                                                      \
+  do {                                                                                                           \
+    if (g.pair)                                                                                                  \
+      hipLaunchKernelGGL((gram_narrow<T, RM_, RN_, true>), grid, block, 0, s, A, lda, ma, B, ldb, nb, w, N,       \
+                         g.kchunk, work);                                                                        \
+    else                                                                                                         \
+      hipLaunchKernelGGL((gram_narrow<T, RM_, RN_, false>), grid, block, 0, s, A, lda, ma, B, ldb, nb, w, N,      \
+                         g.kchunk, work);                                                                        \
   } while (0)
-  if (nb <= 8)
-    IG_NARROW(1);
-  else if (nb <= 16)
-    IG_NARROW(2);
-  else if (nb <= 32)
-    IG_NARROW(4);
+#define IG_NARROW_N(RM_)   \
+  do {                     \
+    if (nb <= 8)           \
+      IG_NARROW(RM_, 1);   \
+    else if (nb <= 16)     \
+      IG_NARROW(RM_, 2);   \
+    else if (nb <= 32)     \
+      IG_NARROW(RM_, 4);   \
+    else                   \
+      IG_NARROW(RM_, 8);   \
+  } while (0)
+  if (ma <= 32)
+    IG_NARROW_N(1);
+  else if (ma <= 64)
+    IG_NARROW_N(2);
+  else if (ma <= 128)
+    IG_NARROW_N(4);
   else
-    IG_NARROW(8);
+    IG_NARROW_N(8);
+#undef IG_NARROW_N
 #undef IG_NARROW
   const long long mn = (long long)m * n;
   hipLaunchKernelGGL(gram_reduce, dim3(reduce_blocks(mn)), dim3(IG_THREADS), 0, s, work, g.nchunk, mn, G, accumulate,

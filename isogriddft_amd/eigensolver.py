@@ -10,21 +10,24 @@ S = W^(1/2) H W^(-1/2) is the operator):
       iterations; in between the basis already is the Ritz basis (C = I, theta from the
       previous step)
     Psi = C^T Z, R = C^T HZ - theta Psi;  stop when max ||R|| <= tol
-    R <- (diag(H) - theta)^-1 R                  (Davidson diagonal preconditioner)
+    R <- M R,  M ~ (T + c)^-1: a few CG steps per row (kinetic preconditioner)
     R <- R - (R Z^T) Z, whitened by the Cholesky factor of R R^T
                                                  (ks/eigensolver_jax.py:119-146)
     HR = S R
     generalized Rayleigh-Ritz on [Z; R]: T c = theta G_B c with T = [[P, Z HR^T],
       [R HZ^T, R HR^T]], G_B the Gram matrix of [Z; R]; lowest m Ritz vectors c
     Z <- c^T [Z; R], HZ <- c^T [HZ; HR]
-Device work per iteration: one H apply of k orbitals (the fused HIP stencil), plain
-products (library GEMMs for wide outputs, the split-K Gram kernel for narrow ones) and the
-basis rotations as library GEMMs; the (m + k)-sized generalized eigenproblems run on the
-host (LAPACK).
+Device work per iteration: one H apply of k orbitals (the fused HIP stencil), the kinetic
+preconditioner (k-row CG: stencil applies and the row-wise CG kernels), the Gram products
+as split-K strided-batched library GEMMs, the basis rotation as a library GEMM, and S Z
+either rotated along (small m) or re-applied (large m); the (m + k)-sized generalized
+eigenproblems run on the host (LAPACK).
 
 The preconditioner is an addition to the reference's block iteration (which has none):
-it changes the iterates, not the Ritz pairs converged to, and cuts the iteration count
-1.6x on the c2 workload and 4x on the legacy 51^3 H2 route.  W is the inner product that
+it changes the iterates, not the Ritz pairs converged to.  The kinetic one cuts the c2
+workload (96^3, k = 16) from 161 iterations with the diagonal preconditioner to 15, and
+makes the 224^3, k = 48 block converge (72 iterations; the diagonal one had not converged
+after 400).  W is the inner product that
 makes H self-adjoint: the cell volumes on monitor grids; on legacy grids the product
 weights (h s_x)(h s_y)(h s_z) of the 7-point flux form, with the returned orbitals then
 re-orthonormalised under the cell volumes as the reference returns them.
@@ -47,7 +50,7 @@ from . import configs
 from .device import DeviceGrid, device_grid
 from .hamiltonian import (BlockHamiltonian, H2_BENCHMARK_CASE, normalize_kinetic_version, normalize_spin_channel,
                           resolve_hartree, resolve_xc)
-from .reductions import gram
+from .reductions import cg_direction, cg_update, gram, row_dots
 
 FP32_RESIDUAL_FLOOR = 5e-5
 
@@ -148,10 +151,48 @@ def _chol_inv(G):
 
 
 PRECONDITIONER_FLOOR = 1e-2
+# kinetic preconditioner: CG steps per residual block and the SPD shift c (Ha)
+KINETIC_CG_STEPS = 6
+KINETIC_SHIFT = 0.5
+# K panel of the split-K Gram products: about NT_PANELS panels, none shorter than NT_MIN_PANEL
+NT_PANELS = 256
+NT_MIN_PANEL = 4096
+
+
+def _gemm_nt_split(A, B):
+    """A B^T (FP64 result) for A (a, N), B (b, N) with unit column stride.  The grid axis
+    is cut into P equal panels that run as one strided-batched library GEMM (a single
+    GEMM with K = N this long keeps only a few workgroups busy), the remainder as one
+    more product, and the partials are added in FP64 in a fixed order.  In FP32 each
+    panel accumulates at most N / P products before the FP64 sum."""
+    a, N = A.shape
+    b = B.shape[0]
+    if A.stride(1) != 1 or B.stride(1) != 1:
+        raise ValueError("Gram operands need unit column stride")
+    kp = max(NT_MIN_PANEL, -(-N // NT_PANELS))
+    kp = -(-kp // 64) * 64
+    P = N // kp
+    if P < 2:
+        return torch.matmul(A, B.T).to(torch.float64)
+    n0 = P * kp
+    A3 = A.as_strided((P, a, kp), (kp, A.stride(0), 1))
+    B3 = B.as_strided((P, kp, b), (kp, 1, B.stride(0)))
+    G = torch.bmm(A3, B3).to(torch.float64).sum(0)
+    if n0 < N:
+        G += torch.matmul(A[:, n0:], B[:, n0:].T).to(torch.float64)
+    return G
+# rows per H apply panel and columns per in-place rotation panel
+APPLY_ROWS = 64
+ROTATION_PANEL = 1 << 20
+# working-subspace size m + k from which S Z is re-applied instead of rotated
+HZ_APPLY_MIN = 400
+# block buffers above this size release the allocator's cached segments first
+BIG_BUFFER_BYTES = 8 << 30
 
 
 def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp32_floor=FP32_RESIDUAL_FLOOR,
-                             keep_basis=False, profile=False, precondition=True):
+                             keep_basis=False, profile=False, precondition="kinetic", inplace_rotation=None,
+                             hz_update=None):
     """Lowest-k eigenpairs of the W-self-adjoint H from the starting block X0 (m, N).
 
     The iteration runs on Z = W^(1/2) X, where the W-inner products become plain ones
@@ -162,15 +203,34 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
     Cholesky factor of its Gram matrix (formed from the small blocks already computed)
     instead of the symmetric inverse square root.
 
-    precondition (default) scales each residual by (diag(H) - theta_i)^-1 before it enters the
-    basis (Davidson's diagonal preconditioner; diag(S) = diag(H) since S is a diagonal
-    similarity of H).  On stretched grids the kinetic diagonal varies over orders of
+    precondition acts on each residual before it enters the basis.  "kinetic" (default,
+    also True) takes KINETIC_CG_STEPS of Jacobi-preconditioned conjugate gradients on
+    (T + c) x = r per residual row (c = KINETIC_SHIFT > 0 so the system is SPD): a
+    polynomial approximation of the inverse kinetic energy that damps the smooth error as
+    well as the rough, at one kinetic block apply of k rows per step.  "diagonal" scales by
+    (diag(H) - theta_i)^-1 (Davidson; diag(S) = diag(H) since S is a diagonal similarity
+    of H).  False / None: no preconditioner (the reference's iteration).  On stretched grids the kinetic diagonal varies over orders of
     magnitude and this removes most of the stiffness; the Ritz pairs it converges to are
     the same.
+
+    inplace_rotation: rotate the basis in column panels inside one buffer instead of
+    alternating two buffers (half the memory, one extra panel copy per rotation).  None
+    (default) takes the two-buffer form when both fit in the free device memory with room
+    to spare, which holds up to about 120 GB of FP64 buffer on one MI355X.
+
+    hz_update: how S Z follows the new basis: "rotate" (HZ' = Q [HR; HZ], 2 m (m + k) N
+    flop) or "apply" (HZ' = S Z', a bandwidth-bound block apply).  None (default) applies
+    once m + k reaches HZ_APPLY_MIN, where the rotation's flops outgrow the apply's bytes.
     """
     if precision not in ("fp64", "mixed"):
         raise ValueError(f"precision must be `fp64` or `mixed`; received `{precision}`.")
+    if hz_update not in (None, "rotate", "apply"):
+        raise ValueError(f"hz_update must be `rotate` or `apply`; received `{hz_update}`.")
     m = X0.shape[0]
+    reapply = hz_update == "apply" or (hz_update is None and m + k >= HZ_APPLY_MIN)
+    if precondition not in (True, False, None, "diagonal", "kinetic"):
+        raise ValueError(f"precondition must be `diagonal`, `kinetic` or False; received `{precondition}`.")
+    pmode = "kinetic" if precondition is True else (precondition or None)
     if k > m:
         raise ValueError("the starting block must hold at least k orbitals")
     shape = H.dg.shape
@@ -179,7 +239,7 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
     w64 = w.reshape(-1).to(torch.float64)
     sw64 = torch.sqrt(w64)
     dt = torch.float32 if precision == "mixed" else torch.float64
-    tm = {"hamiltonian_apply": 0.0, "gram": 0.0, "eigh": 0.0, "rotation": 0.0}
+    tm = {"hamiltonian_apply": 0.0, "gram": 0.0, "eigh": 0.0, "rotation": 0.0, "precondition": 0.0}
     clock = [time.perf_counter()]
 
     def mark(key):
@@ -195,40 +255,110 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
         return sw, 1.0 / sw
 
     sw, isw = setup(dt)
-    hdiag = H.diagonal() if precondition else None
+    hdiag = H.diagonal() if pmode == "diagonal" else None
+    if pmode == "kinetic":
+        kop = BlockHamiltonian(H.dg, torch.full((N,), KINETIC_SHIFT, dtype=torch.float64, device=dev),
+                               closure=H.closure)
+        kdinv64 = 1.0 / (-0.5 * H.dg.kinetic_diagonal() + KINETIC_SHIFT)
 
-    def apply_s(Zb, out):
-        """out = S Zb = W^(1/2) H W^(-1/2) Zb for a block of rows."""
+    def apply_s(Zb, out, op=H, phase="hamiltonian_apply"):
+        """out = S Zb = W^(1/2) H W^(-1/2) Zb for a block of rows (in panels of at most
+        APPLY_ROWS rows, so the unwhitened temporary stays small)."""
         mark("rotation")
-        xb = (Zb * isw).view((Zb.shape[0],) + shape)
-        H.apply(xb, out.view((Zb.shape[0],) + shape))
-        out.mul_(sw)
-        mark("hamiltonian_apply")
+        for r0 in range(0, Zb.shape[0], APPLY_ROWS):
+            r1 = min(r0 + APPLY_ROWS, Zb.shape[0])
+            xb = (Zb[r0:r1] * isw).view((r1 - r0,) + shape)
+            op.apply(xb, out[r0:r1].view((r1 - r0,) + shape))
+            out[r0:r1].mul_(sw)
+        mark(phase)
+
+    cg_bufs = {}
+
+    def kinetic_cg(Rb):
+        """Rb <- ~ (S_T + c)^-1 Rb row by row.  The CG runs on the X-space rows
+        x = W^(-1/2) r with the W-self-adjoint T + c and W-inner products (no similarity
+        scaling around each stencil apply): KINETIC_CG_STEPS of Jacobi-preconditioned CG
+        from x0 = D^-1 r, D = diag(T) + c, then the result is mapped back by W^(1/2)."""
+        dtb = Rb.dtype
+        if dtb not in cg_bufs:
+            cg_bufs.clear()
+            cg_bufs[dtb] = (torch.empty((4, k, N), dtype=dtb, device=dev), kdinv64.to(dtb), w64.to(dtb))
+        buf, dinv, wd = cg_bufs[dtb]
+        x, r, pv, q = buf[0], buf[1], buf[2], buf[3]
+        mark("rotation")
+        torch.mul(Rb, isw, out=r)                       # r = W^(-1/2) R
+        torch.mul(r, dinv, out=x)                       # x0 = D^-1 r
+        kop.apply(x.view((k,) + shape), q.view((k,) + shape))
+        r.sub_(q)
+        torch.mul(r, dinv, out=pv)
+        rz = row_dots(r, pv, wd)
+        for _ in range(KINETIC_CG_STEPS):
+            kop.apply(pv.view((k,) + shape), q.view((k,) + shape))
+            pq = row_dots(pv, q, wd)
+            alpha = torch.where(pq > 0, rz / torch.where(pq > 0, pq, 1.0), 0.0)
+            rz_new = cg_update(x, r, pv, q, dinv, wd, alpha)
+            beta = torch.where(rz > 0, rz_new / torch.where(rz > 0, rz, 1.0), 0.0)
+            cg_direction(pv, r, dinv, beta)
+            rz = rz_new
+        torch.mul(x, sw, out=Rb)
+        mark("precondition")
 
     def gemm_nt(A, B):
-        """A B^T as FP64.  Products with one side of at most 64 rows and the other of at
-        most 256 go to the split-K Gram kernels (a library GEMM without a K split runs
-        them on a few workgroups); wider ones to the library GEMM in the block dtype."""
-        a, b = A.shape[0], B.shape[0]
-        if min(a, b) <= 64 and max(a, b) <= 256:
-            return gram(A, B, None)
-        return torch.matmul(A, B.T).to(torch.float64)
+        """A B^T as FP64 (see _gemm_nt_split)."""
+        return _gemm_nt_split(A, B)
 
-    def orthonormal_start(Z):
+    def orthonormalize_rows(Z):
+        """Z <- L^-1 Z with Z Z^T = L L^T, in place (column panels)."""
         G = gemm_nt(Z, Z)
         Linv = _chol_inv(G).to(dev)
-        return torch.matmul(Linv.to(Z.dtype), Z)
+        rotate(Linv.to(Z.dtype), Z, Z, None)
 
     # One buffer holds the rows [Z; R; HR; HZ] (basis, residuals, S R, S Z), so [Z; R],
     # [R; HR] and [HR; HZ] are contiguous: each basis rotation is one GEMM and one pass
     # of the Gram kernel gives Z R^T, Z HR^T, R R^T and R HR^T together.  Two buffers
-    # alternate between iterations.
+    # alternate between iterations, or one is rotated in place when two do not fit.
     def views(B):
         return B[:m], B[m:m + k], B[m + k:m + 2 * k], B[m + 2 * k:]
 
-    BUF = torch.empty((2 * m + 2 * k, N), dtype=dt, device=dev)
-    BUF_next = torch.empty_like(BUF)
-    BUF[:m].copy_(orthonormal_start((X0.reshape(m, N).to(dt) * sw)))
+    rows = 2 * m + 2 * k
+
+    def second_buffer(dtype):
+        """A second buffer for the out-of-place rotations, or None for in-place panels."""
+        if inplace_rotation is True:
+            return None
+        nbytes = rows * N * torch.finfo(dtype).bits // 8
+        if inplace_rotation is None and dev.type == "cuda":
+            free, _ = torch.cuda.mem_get_info(dev)
+            free += torch.cuda.memory_reserved(dev) - torch.cuda.memory_allocated(dev)  # cached blocks
+            if nbytes > 0.45 * free:
+                return None
+        return torch.empty((rows, N), dtype=dtype, device=dev)
+
+    def rotate(Qm, src, dst, spare):
+        """dst = Qm src, out of place into the spare buffer's rows or in column panels."""
+        if spare is not None:
+            torch.matmul(Qm, src, out=dst)
+            return
+        for c0 in range(0, N, ROTATION_PANEL):
+            c1 = min(c0 + ROTATION_PANEL, N)
+            dst[:, c0:c1].copy_(torch.matmul(Qm, src[:, c0:c1]))
+
+    if rows * N * 8 > BIG_BUFFER_BYTES and dev.type == "cuda":
+        torch.cuda.empty_cache()  # whole free segments back, so one large buffer fits
+    store = None
+    if dt == torch.float32:
+        # one FP64-sized storage: its two halves are the alternating FP32 buffers, and
+        # after the promotion the FP64 buffer (no allocation at the promotion)
+        store = torch.empty(rows * N, dtype=torch.float64, device=dev)
+        halves = store.view(torch.float32)
+        BUF = halves[rows * N:].view(rows, N)
+        BUF_next = None if inplace_rotation is True else halves[:rows * N].view(rows, N)
+        BUF[:m].copy_(X0.reshape(m, N) * sw64)
+    else:
+        BUF = torch.empty((rows, N), dtype=dt, device=dev)
+        torch.mul(X0.reshape(m, N), sw64, out=BUF[:m])
+        BUF_next = second_buffer(dt)
+    orthonormalize_rows(BUF[:m])
     apply_s(BUF[:m], BUF[m + 2 * k:])
 
     res = BlockIterationResult(np.zeros(k), BUF[:k], np.full(k, np.inf), 0, False)
@@ -281,12 +411,20 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
                 phase32 = False
                 dt = torch.float64
                 sw, isw = setup(dt)
-                Z64 = orthonormal_start(Z.to(torch.float64))
-                del BUF_next
-                BUF = torch.empty((2 * m + 2 * k, N), dtype=dt, device=dev)
-                BUF_next = torch.empty_like(BUF)
-                BUF[:m].copy_(Z64)
-                del Z64
+                # The FP32 basis goes to the storage's second half (whose rows [0, m) lie
+                # past the bytes of the FP64 rows [0, m) since 8 m N < 4 (2m + 2k) N), is
+                # widened into the FP64 buffer over the same storage, then re-orthonormalised.
+                res.orbitals = Psi.to(torch.float64)
+                half2 = halves[rows * N:].view(rows, N)
+                if BUF.data_ptr() != half2.data_ptr():
+                    half2[:m].copy_(Z)
+                del Z, R, HR, HZ, Psi
+                BUF = BUF_next = None
+                BUF = store.view(rows, N)
+                BUF[:m].copy_(half2[:m])
+                del half2
+                orthonormalize_rows(BUF[:m])
+                BUF_next = second_buffer(dt)
                 apply_s(BUF[:m], BUF[m + 2 * k:])
                 refresh = True
                 continue
@@ -298,6 +436,8 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
         if hdiag is not None:
             den = hdiag[None, :] - torch.from_numpy(th).to(dev)[:, None]
             R.div_(den.clamp_min_(PRECONDITIONER_FLOOR).to(dt))
+        elif pmode == "kinetic":
+            kinetic_cg(R)
         # project the residuals out of the basis and whiten them
         GRZ = gemm_nt(R, Z)
         mark("gram")
@@ -323,10 +463,18 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
         Q = c[:, :m].T.contiguous().to(dev)  # new basis = Q [Z; R]
         mark("eigh")
         Qd = Q.to(dt)
-        torch.matmul(Qd, BUF[:m + k], out=BUF_next[:m])               # Z' = Q [Z; R]
-        Qh = torch.cat([Qd[:, m:], Qd[:, :m]], 1)
-        torch.matmul(Qh, BUF[m + k:], out=BUF_next[m + 2 * k:])        # HZ' = Q [HR; HZ] reordered
-        BUF, BUF_next = BUF_next, BUF
+        out = BUF_next if BUF_next is not None else BUF
+        # Z' = Q [Z; R] and HZ' = Q [HR; HZ] (columns of Q reordered).  In place, each
+        # column panel of the sources is read whole before that panel is written, and the
+        # two rotations touch disjoint rows ([0, m + k) -> [0, m), [m + k, end) -> [m + 2k, end)).
+        rotate(Qd, BUF[:m + k], out[:m], BUF_next)
+        if not reapply:
+            Qh = torch.cat([Qd[:, m:], Qd[:, :m]], 1)
+            rotate(Qh, BUF[m + k:], out[m + 2 * k:], BUF_next)
+        if BUF_next is not None:
+            BUF, BUF_next = BUF_next, BUF
+        if reapply:
+            apply_s(BUF[:m], BUF[m + 2 * k:])
         res.eigenvalues, res.orbitals, res.residual_norms = th, Psi, rn
         res.iterations = it
         res.ritz_matrix = ritz_k

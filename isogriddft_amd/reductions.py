@@ -87,3 +87,55 @@ def gram(X: torch.Tensor, Y: torch.Tensor, w: torch.Tensor, out: torch.Tensor | 
               _lib.ptr(ww), N, _lib.ptr(out), int(accumulate), _lib.ptr(work), work.numel(),
               _lib.stream_handle(stream))
     return out
+
+
+# ------------------------------------------------------------------ row-wise block CG
+
+
+def _rows(t):
+    if t.dim() != 2 or t.stride(1) != 1:
+        raise ValueError("row-wise kernels take (k, N) blocks with unit column stride")
+    return t
+
+
+def row_dots(a: torch.Tensor, b: torch.Tensor, w: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+    """FP64 device vector out[i] = sum_n w[n] a[i, n] b[i, n] (w None: 1)."""
+    a, b = _rows(a), _rows(b)
+    k, N = a.shape
+    if b.shape != a.shape or b.stride(0) != a.stride(0) or b.dtype != a.dtype:
+        raise ValueError("row_dots operands must share shape, row stride and dtype")
+    lib = _lib.load()
+    work = workspace(a.device, lib.ig_row_dots_work_size(k), "rows")
+    out = torch.empty(k, dtype=torch.float64, device=a.device)
+    sfx = "f64" if a.dtype == torch.float64 else "f32"
+    _lib.call(f"ig_row_dots_{sfx}", _lib.ptr(a), _lib.ptr(b), _lib.ptr(w), k, N, a.stride(0), _lib.ptr(work),
+              _lib.ptr(out), _lib.stream_handle(stream))
+    return out
+
+
+def cg_update(x, r, p, q, dinv, w, alpha, stream=None) -> torch.Tensor:
+    """Row i: x += alpha_i p, r -= alpha_i q; returns the FP64 vector sum w dinv r_i^2."""
+    for t in (x, r, p, q):
+        _rows(t)
+        if t.shape != x.shape or t.stride(0) != x.stride(0) or t.dtype != x.dtype:
+            raise ValueError("cg_update blocks must share shape, row stride and dtype")
+    k, N = x.shape
+    lib = _lib.load()
+    work = workspace(x.device, lib.ig_row_dots_work_size(k), "rows")
+    out = torch.empty(k, dtype=torch.float64, device=x.device)
+    sfx = "f64" if x.dtype == torch.float64 else "f32"
+    _lib.call(f"ig_cg_update_{sfx}", _lib.ptr(x), _lib.ptr(r), _lib.ptr(p), _lib.ptr(q), _lib.ptr(dinv),
+              _lib.ptr(w), _lib.ptr(alpha), k, N, x.stride(0), _lib.ptr(work), _lib.ptr(out),
+              _lib.stream_handle(stream))
+    return out
+
+
+def cg_direction(p, r, dinv, beta, stream=None) -> None:
+    """Row i: p = dinv r + beta_i p."""
+    _rows(p), _rows(r)
+    k, N = p.shape
+    if r.shape != p.shape or r.stride(0) != p.stride(0) or r.dtype != p.dtype:
+        raise ValueError("cg_direction blocks must share shape, row stride and dtype")
+    sfx = "f64" if p.dtype == torch.float64 else "f32"
+    _lib.call(f"ig_cg_direction_{sfx}", _lib.ptr(p), _lib.ptr(r), _lib.ptr(dinv), _lib.ptr(beta), k, N,
+              p.stride(0), _lib.stream_handle(stream))

@@ -40,16 +40,17 @@ def _start(dg, g, m):
     return eigensolver.default_guess_block(g, atoms, m, dg)
 
 
+@pytest.mark.parametrize("precondition", ["diagonal", "kinetic"])
 @pytest.mark.parametrize("k", [1, 3])
-def test_block_iteration_matches_dense_eigh(k):
+def test_block_iteration_matches_dense_eigh(k, precondition):
     g, V = _problem()
     dg = DeviceGrid(g)
     H = BlockHamiltonian(dg, torch.from_numpy(V).cuda(), closure="trial_fix")
     m = eigensolver.reference_block_size(k)
     X0 = _start(dg, g, m)
-    res = eigensolver.block_subspace_iteration(H, dg.weights(), X0, k, 1e-7, 400)
+    res = eigensolver.block_subspace_iteration(H, dg.weights(), X0, k, 1e-7, 400, precondition=precondition)
     ref = _dense_eigs(g, V, k)
-    print("k", k, "iters", res.iterations, res.eigenvalues - ref)
+    print("k", k, precondition, "iters", res.iterations, res.eigenvalues - ref)
     assert res.converged
     assert np.max(np.abs(res.eigenvalues - ref)) <= 1e-8
     psi = res.orbitals.reshape(k, -1)
@@ -57,19 +58,61 @@ def test_block_iteration_matches_dense_eigh(k):
     assert torch.max(torch.abs(S - torch.eye(k, device=S.device, dtype=S.dtype))).item() <= 1e-12
 
 
-def test_mixed_precision_refinement_matches_fp64():
+@pytest.mark.parametrize("precondition", ["diagonal", "kinetic"])
+def test_mixed_precision_refinement_matches_fp64(precondition):
     g, V = _problem((14, 15, 16))
     dg = DeviceGrid(g)
     H = BlockHamiltonian(dg, torch.from_numpy(V).cuda(), closure="trial_fix")
     k = 3
     X0 = _start(dg, g, eigensolver.reference_block_size(k))
-    r64 = eigensolver.block_subspace_iteration(H, dg.weights(), X0, k, 1e-6, 400)
-    rmx = eigensolver.block_subspace_iteration(H, dg.weights(), X0, k, 1e-6, 400, precision="mixed")
+    r64 = eigensolver.block_subspace_iteration(H, dg.weights(), X0, k, 1e-6, 400, precondition=precondition)
+    rmx = eigensolver.block_subspace_iteration(H, dg.weights(), X0, k, 1e-6, 400, precision="mixed",
+                                               precondition=precondition)
     print("fp64 iters", r64.iterations, "mixed", rmx.iterations, "fp32 part", rmx.fp32_iterations,
           rmx.eigenvalues - r64.eigenvalues)
     assert r64.converged and rmx.converged and rmx.fp32_iterations > 0
     assert np.max(np.abs(rmx.eigenvalues - r64.eigenvalues)) <= 1e-6
     assert rmx.orbitals.dtype == torch.float64
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_inplace_rotation_matches_two_buffers(monkeypatch, precision):
+    """The single-buffer form (column panels rotated in place) gives the two-buffer
+    iteration's eigenvalues; a small panel width forces many panels and a ragged last one."""
+    g, V = _problem((14, 15, 16))
+    dg = DeviceGrid(g)
+    H = BlockHamiltonian(dg, torch.from_numpy(V).cuda(), closure="trial_fix")
+    k = 3
+    X0 = _start(dg, g, eigensolver.reference_block_size(k))
+    ref = eigensolver.block_subspace_iteration(H, dg.weights(), X0, k, 1e-6, 400, precision=precision,
+                                               inplace_rotation=False)
+    monkeypatch.setattr(eigensolver, "ROTATION_PANEL", 1000)
+    monkeypatch.setattr(eigensolver, "APPLY_ROWS", 7)
+    got = eigensolver.block_subspace_iteration(H, dg.weights(), X0, k, 1e-6, 400, precision=precision,
+                                               inplace_rotation=True)
+    print(precision, ref.iterations, got.iterations, got.eigenvalues - ref.eigenvalues)
+    assert ref.converged and got.converged
+    assert np.max(np.abs(got.eigenvalues - ref.eigenvalues)) <= 1e-9
+    assert got.orbitals.dtype == torch.float64
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_hz_reapply_matches_rotation(precision):
+    """S Z re-applied after each rotation gives the eigenvalues of the rotated S Z."""
+    g, V = _problem((14, 15, 16))
+    dg = DeviceGrid(g)
+    H = BlockHamiltonian(dg, torch.from_numpy(V).cuda(), closure="trial_fix")
+    k = 3
+    X0 = _start(dg, g, eigensolver.reference_block_size(k))
+    ref = eigensolver.block_subspace_iteration(H, dg.weights(), X0, k, 1e-6, 400, precision=precision,
+                                               hz_update="rotate")
+    got = eigensolver.block_subspace_iteration(H, dg.weights(), X0, k, 1e-6, 400, precision=precision,
+                                               hz_update="apply")
+    print(precision, ref.iterations, got.iterations, got.eigenvalues - ref.eigenvalues)
+    assert ref.converged and got.converged
+    assert np.max(np.abs(got.eigenvalues - ref.eigenvalues)) <= 1e-8
+    with pytest.raises(ValueError):
+        eigensolver.block_subspace_iteration(H, dg.weights(), X0, k, 1e-6, 4, hz_update="bogus")
 
 
 def test_orthonormalize_spans_oracle_subspace():

@@ -75,3 +75,63 @@ def test_gram_narrow_shapes(dtype, m, n, N, weighted):
     assert np.max(np.abs(G.cpu().numpy() - ref) / scale) < 1e-12
     G2 = reductions.gram(t(X), t(Y), t(w), out=G.clone(), accumulate=True)
     assert np.max(np.abs(G2.cpu().numpy() - 2 * ref) / scale) < 1e-12
+
+
+@pytest.mark.parametrize("dt", [torch.float64, torch.float32])
+def test_split_k_nt_products(dt):
+    """The eigensolver's split-K A B^T (strided-batched panels plus a tail) on row views of
+    one buffer, against an FP64 product."""
+    from isogriddft_amd.eigensolver import _gemm_nt_split
+
+    N = 20000  # four 4096-wide panels and a 3616-wide tail
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    buf = torch.randn((70, N), dtype=torch.float64, device="cuda", generator=g).to(dt)
+    for (a0, a1, b0, b1) in ((0, 40, 40, 70), (5, 65, 0, 12), (0, 70, 0, 70)):
+        A, B = buf[a0:a1], buf[b0:b1]
+        got = _gemm_nt_split(A, B)
+        want = A.double() @ B.double().T
+        err = float((got - want).abs().max() / want.abs().max())
+        assert got.dtype == torch.float64 and got.shape == (a1 - a0, b1 - b0)
+        assert err <= (1e-12 if dt == torch.float64 else 2e-6), err
+
+
+@pytest.mark.parametrize("dt", [torch.float64, torch.float32])
+def test_row_wise_cg_kernels(dt):
+    """Row dots, the fused CG update and the direction update against torch on a ragged
+    (row stride > N) block; FP64 sums (the FP32 case compares with FP64 sums of the same
+    FP32 values)."""
+    from isogriddft_amd.reductions import cg_direction, cg_update, row_dots
+
+    k, N, ld = 5, 70001, 70016
+    g = torch.Generator(device="cuda")
+    g.manual_seed(9)
+
+    def block():
+        return torch.randn((k, ld), dtype=torch.float64, device="cuda", generator=g).to(dt)[:, :N]
+
+    x, r, p, q = block(), block(), block(), block()
+    w = torch.rand(N, dtype=torch.float64, device="cuda", generator=g).to(dt) + 0.5
+    dinv = torch.rand(N, dtype=torch.float64, device="cuda", generator=g).to(dt) + 0.1
+    tol = 1e-13 if dt == torch.float64 else 1e-6
+    d = row_dots(p, q, w)
+    want = (p.double() * q.double() * w.double()).sum(1)
+    assert torch.allclose(d, want, rtol=tol, atol=tol * want.abs().max().item())
+    assert torch.equal(row_dots(p, q, w), d)  # deterministic
+    d0 = row_dots(p, q, None)
+    assert torch.allclose(d0, (p.double() * q.double()).sum(1), rtol=tol, atol=tol * N)
+    alpha = torch.linspace(-0.7, 0.9, k, dtype=torch.float64, device="cuda")
+    x0, r0 = x.clone(), r.clone()
+    rz = cg_update(x, r, p, q, dinv, w, alpha)
+    a = alpha.to(dt)[:, None]
+    xw = x0 + a * p
+    rw = r0 - a * q
+    assert torch.allclose(x, xw, rtol=0, atol=4 * torch.finfo(dt).eps * xw.abs().max().item())
+    assert torch.allclose(r, rw, rtol=0, atol=4 * torch.finfo(dt).eps * rw.abs().max().item())
+    rzw = (r.double() ** 2 * dinv.double() * w.double()).sum(1)
+    assert torch.allclose(rz, rzw, rtol=tol, atol=0)
+    beta = torch.linspace(0.1, 1.3, k, dtype=torch.float64, device="cuda")
+    p0 = p.clone()
+    cg_direction(p, r, dinv, beta)
+    pw = dinv * r + beta.to(dt)[:, None] * p0
+    assert torch.allclose(p, pw, rtol=0, atol=4 * torch.finfo(dt).eps * pw.abs().max().item())
