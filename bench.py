@@ -49,7 +49,10 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-solvers", action="store_true",
                     help="skip the eigensolve + Poisson timing (FP64 vs FP32 + FP64 refinement)")
-    ap.add_argument("--eig-tol", type=float, default=1e-3)
+    ap.add_argument("--eig-tol", type=float, default=None,
+                    help="eigensolver residual tolerance (default: 1e-3 for c2, 3e-4 for c5)")
+    ap.add_argument("--eig-block", type=int, default=None,
+                    help="working-subspace size (default: the reference's 12k + 8 for c2, 4k + 8 for c5)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (the scale for PMC byte counters)")
     return ap.parse_args()
@@ -86,10 +89,11 @@ def cpu_baseline(geom, psi_np, veff_np, seconds):
     }
 
 
-def solver_leg(geom, dg, cfg, veff64, eig_tol, order=2):
+def solver_leg(geom, dg, cfg, veff64, eig_tol, order=2, block=None):
     """Poisson (multipoles of the given order, rtol 1e-8) on the synthetic charges, then the lowest-k
-    eigenproblem of T + V + v_H (production closure, working subspace 12k + 8, residual
-    tolerance eig_tol), each in FP64 and in FP32 with FP64 refinement."""
+    eigenproblem of T + V + v_H (trial_fix closure, working subspace `block`, default the
+    reference's 12k + 8, residual tolerance eig_tol), each in FP64 and in FP32 with FP64
+    refinement."""
     import torch
 
     from isogriddft_amd import configs, eigensolver, poisson
@@ -118,7 +122,7 @@ def solver_leg(geom, dg, cfg, veff64, eig_tol, order=2):
                for c in cfg.centers)
     H = BlockHamiltonian(dg, vion + pots["fp64"].reshape(-1), closure="trial_fix")
     k = cfg.k
-    m = eigensolver.reference_block_size(k)
+    m = block or eigensolver.reference_block_size(k)
     atoms = [("H", c) for c in cfg.centers]
     X0 = eigensolver.default_guess_block(geom, atoms, m, dg)
     print(f"[bench] starting block of {m} orbitals built", file=sys.stderr, flush=True)
@@ -252,8 +256,10 @@ def main():
     del ins, outs, psi0
     torch.cuda.empty_cache()
     if not args.no_solvers:
-        line["eigensolve_poisson"] = solver_leg(geom, dg, cfg, veff64, args.eig_tol,
-                                                order=2 if args.workload == "c2" else 4)
+        c2 = args.workload == "c2"
+        tol = args.eig_tol or (1e-3 if c2 else 3e-4)
+        block = args.eig_block or (None if c2 else 4 * k + 8)
+        line["eigensolve_poisson"] = solver_leg(geom, dg, cfg, veff64, tol, order=2 if c2 else 4, block=block)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(geom, psi_np, veff64.cpu().numpy().reshape(dg.shape), args.cpu_seconds)
     print(json.dumps(line), flush=True)

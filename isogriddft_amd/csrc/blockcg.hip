@@ -11,9 +11,129 @@
 // fixed order, so results are bit-reproducible.
 #include "common.h"
 
+#include <initializer_list>
+
 namespace {
 
 constexpr int CG_MAX_CHUNKS = 256;
+
+// 16-byte vectors: float4 / double2.  The vector kernels run when N, ld and the chunk are
+// multiples of VW and every base pointer is 16-byte aligned (the host checks), otherwise
+// the scalar ones.
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  typedef float t __attribute__((ext_vector_type(4)));
+  static constexpr int W = 4;
+};
+template <>
+struct Vec16<double> {
+  typedef double t __attribute__((ext_vector_type(2)));
+  static constexpr int W = 2;
+};
+
+template <typename T>
+__device__ __forceinline__ double vsum_prod(const typename Vec16<T>::t& a, const typename Vec16<T>::t& b) {
+  double s = 0.0;
+#pragma unroll
+  for (int e = 0; e < Vec16<T>::W; ++e) s += (double)a[e] * (double)b[e];
+  return s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(IG_THREADS) void row_dots_vec(const T* __restrict__ a, const T* __restrict__ b,
+                                                           const T* __restrict__ w, long long N, long long ld,
+                                                           long long chunk, double* __restrict__ partial) {
+  using VT = typename Vec16<T>::t;
+  constexpr int VW = Vec16<T>::W;
+  __shared__ double sm[4];
+  const int row = blockIdx.y;
+  const long long beg = (long long)blockIdx.x * chunk;
+  const long long end = min(beg + chunk, N);
+  const T* ar = a + row * ld;
+  const T* br = b + row * ld;
+  double s = 0.0;
+  for (long long n = beg + (long long)threadIdx.x * VW; n < end; n += (long long)IG_THREADS * VW) {
+    const VT va = *reinterpret_cast<const VT*>(ar + n);
+    const VT vb = *reinterpret_cast<const VT*>(br + n);
+    if (w) {
+      const VT vw = *reinterpret_cast<const VT*>(w + n);
+#pragma unroll
+      for (int e = 0; e < VW; ++e) s += (double)va[e] * (double)vb[e] * (double)vw[e];
+    } else {
+      s += vsum_prod<T>(va, vb);
+    }
+  }
+  double v[1] = {s};
+  ig_block_sum<1>(v, sm);
+  if (threadIdx.x == 0) partial[(long long)row * gridDim.x + blockIdx.x] = v[0];
+}
+
+template <typename T>
+__global__ __launch_bounds__(IG_THREADS) void cg_update_vec(T* __restrict__ x, T* __restrict__ r,
+                                                            const T* __restrict__ p, const T* __restrict__ q,
+                                                            const T* __restrict__ dinv, const T* __restrict__ w,
+                                                            const double* __restrict__ alpha, long long N,
+                                                            long long ld, long long chunk,
+                                                            double* __restrict__ partial) {
+  using VT = typename Vec16<T>::t;
+  constexpr int VW = Vec16<T>::W;
+  __shared__ double sm[4];
+  const int row = blockIdx.y;
+  const long long beg = (long long)blockIdx.x * chunk;
+  const long long end = min(beg + chunk, N);
+  const T a = (T)alpha[row];
+  T* xr = x + row * ld;
+  T* rr = r + row * ld;
+  const T* pr = p + row * ld;
+  const T* qr = q + row * ld;
+  double s = 0.0;
+  for (long long n = beg + (long long)threadIdx.x * VW; n < end; n += (long long)IG_THREADS * VW) {
+    VT vx = *reinterpret_cast<const VT*>(xr + n);
+    VT vr = *reinterpret_cast<const VT*>(rr + n);
+    const VT vp = *reinterpret_cast<const VT*>(pr + n);
+    const VT vq = *reinterpret_cast<const VT*>(qr + n);
+    const VT vd = *reinterpret_cast<const VT*>(dinv + n);
+    VT vw;
+    if (w) vw = *reinterpret_cast<const VT*>(w + n);
+#pragma unroll
+    for (int e = 0; e < VW; ++e) {
+      vx[e] = vx[e] + a * vp[e];
+      vr[e] = vr[e] - a * vq[e];
+      const double t = (double)vr[e] * (double)vr[e] * (double)vd[e];
+      s += w ? t * (double)vw[e] : t;
+    }
+    *reinterpret_cast<VT*>(xr + n) = vx;
+    *reinterpret_cast<VT*>(rr + n) = vr;
+  }
+  double v[1] = {s};
+  ig_block_sum<1>(v, sm);
+  if (threadIdx.x == 0) partial[(long long)row * gridDim.x + blockIdx.x] = v[0];
+}
+
+template <typename T>
+__global__ __launch_bounds__(IG_THREADS) void cg_direction_vec(T* __restrict__ p, const T* __restrict__ r,
+                                                               const T* __restrict__ dinv,
+                                                               const double* __restrict__ beta, long long N,
+                                                               long long ld, long long chunk) {
+  using VT = typename Vec16<T>::t;
+  constexpr int VW = Vec16<T>::W;
+  const int row = blockIdx.y;
+  const long long beg = (long long)blockIdx.x * chunk;
+  const long long end = min(beg + chunk, N);
+  const T b = (T)beta[row];
+  T* pr = p + row * ld;
+  const T* rr = r + row * ld;
+  for (long long n = beg + (long long)threadIdx.x * VW; n < end; n += (long long)IG_THREADS * VW) {
+    VT vp = *reinterpret_cast<const VT*>(pr + n);
+    const VT vr = *reinterpret_cast<const VT*>(rr + n);
+    const VT vd = *reinterpret_cast<const VT*>(dinv + n);
+#pragma unroll
+    for (int e = 0; e < VW; ++e) vp[e] = vd[e] * vr[e] + b * vp[e];
+    *reinterpret_cast<VT*>(pr + n) = vp;
+  }
+}
 
 template <typename T>
 __global__ __launch_bounds__(IG_THREADS) void row_dots_kernel(const T* __restrict__ a, const T* __restrict__ b,
@@ -91,7 +211,7 @@ struct RowPlan {
   long long chunk;
 };
 
-RowPlan row_plan(int k, long long N) {
+RowPlan row_plan(int k, long long N, int align) {
   long long nch = (2048 + k - 1) / k;
   const long long cap = (N + IG_THREADS - 1) / IG_THREADS;  // at least one element per thread
   if (nch > cap) nch = cap;
@@ -99,16 +219,32 @@ RowPlan row_plan(int k, long long N) {
   if (nch < 1) nch = 1;
   RowPlan g;
   g.chunk = (N + nch - 1) / nch;
+  g.chunk = (g.chunk + align - 1) / align * align;
   g.nch = (int)((N + g.chunk - 1) / g.chunk);
   return g;
+}
+
+inline bool al16(const void* q) { return q == nullptr || (reinterpret_cast<uintptr_t>(q) & 15u) == 0; }
+
+template <typename T>
+inline bool use_vec(long long N, long long ld, std::initializer_list<const void*> ptrs) {
+  constexpr int VW = Vec16<T>::W;
+  if (N % VW != 0 || ld % VW != 0) return false;
+  for (const void* q : ptrs)
+    if (!al16(q)) return false;
+  return true;
 }
 
 template <typename T>
 int row_dots(const T* a, const T* b, const T* w, int k, long long N, long long ld, double* work, double* out,
              hipStream_t s) {
   if (!a || !b || !work || !out || k <= 0 || k > 65535 || N <= 0 || ld < N) return IG_EINVAL;
-  const RowPlan g = row_plan(k, N);
-  hipLaunchKernelGGL(row_dots_kernel<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, a, b, w, N, ld, g.chunk, work);
+  const bool vec = use_vec<T>(N, ld, {a, b, w});
+  const RowPlan g = row_plan(k, N, vec ? Vec16<T>::W : 1);
+  if (vec)
+    hipLaunchKernelGGL(row_dots_vec<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, a, b, w, N, ld, g.chunk, work);
+  else
+    hipLaunchKernelGGL(row_dots_kernel<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, a, b, w, N, ld, g.chunk, work);
   hipLaunchKernelGGL(rows_finish, dim3((k + 255) / 256), dim3(256), 0, s, work, g.nch, k, out);
   IG_CHECK_LAUNCH();
   return IG_OK;
@@ -119,9 +255,14 @@ int cg_update(T* x, T* r, const T* p, const T* q, const T* dinv, const T* w, con
               long long ld, double* work, double* out, hipStream_t s) {
   if (!x || !r || !p || !q || !dinv || !alpha || !work || !out || k <= 0 || k > 65535 || N <= 0 || ld < N)
     return IG_EINVAL;
-  const RowPlan g = row_plan(k, N);
-  hipLaunchKernelGGL(cg_update_kernel<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, x, r, p, q, dinv, w, alpha, N, ld,
-                     g.chunk, work);
+  const bool vec = use_vec<T>(N, ld, {x, r, p, q, dinv, w});
+  const RowPlan g = row_plan(k, N, vec ? Vec16<T>::W : 1);
+  if (vec)
+    hipLaunchKernelGGL(cg_update_vec<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, x, r, p, q, dinv, w, alpha, N, ld,
+                       g.chunk, work);
+  else
+    hipLaunchKernelGGL(cg_update_kernel<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, x, r, p, q, dinv, w, alpha, N,
+                       ld, g.chunk, work);
   hipLaunchKernelGGL(rows_finish, dim3((k + 255) / 256), dim3(256), 0, s, work, g.nch, k, out);
   IG_CHECK_LAUNCH();
   return IG_OK;
@@ -131,9 +272,14 @@ template <typename T>
 int cg_direction(T* p, const T* r, const T* dinv, const double* beta, int k, long long N, long long ld,
                  hipStream_t s) {
   if (!p || !r || !dinv || !beta || k <= 0 || k > 65535 || N <= 0 || ld < N) return IG_EINVAL;
-  const RowPlan g = row_plan(k, N);
-  hipLaunchKernelGGL(cg_direction_kernel<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, p, r, dinv, beta, N, ld,
-                     g.chunk);
+  const bool vec = use_vec<T>(N, ld, {p, r, dinv});
+  const RowPlan g = row_plan(k, N, vec ? Vec16<T>::W : 1);
+  if (vec)
+    hipLaunchKernelGGL(cg_direction_vec<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, p, r, dinv, beta, N, ld,
+                       g.chunk);
+  else
+    hipLaunchKernelGGL(cg_direction_kernel<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, p, r, dinv, beta, N, ld,
+                       g.chunk);
   IG_CHECK_LAUNCH();
   return IG_OK;
 }

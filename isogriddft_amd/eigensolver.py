@@ -50,6 +50,7 @@ from . import configs
 from .device import DeviceGrid, device_grid
 from .hamiltonian import (BlockHamiltonian, H2_BENCHMARK_CASE, normalize_kinetic_version, normalize_spin_channel,
                           resolve_hartree, resolve_xc)
+from .kinetic import apply_operator
 from .reductions import cg_direction, cg_update, gram, row_dots
 
 FP32_RESIDUAL_FLOOR = 5e-5
@@ -130,12 +131,24 @@ def _host_geigh(A, B):
     """Generalized symmetric eigenproblem A c = theta B c (B SPD) on the host: ascending
     theta and c with c^T B c = I, by Cholesky reduction (torch's LAPACK: a second BLAS
     thread pool in the same process, e.g. SciPy's, was measured to slow these small
-    solves down by 50x)."""
+    solves down by 50x).  If B is numerically singular (a direction of the expansion
+    lies in the span of the others to rounding), the problem is solved on the range of
+    B instead: B = U diag(l) U^T, directions with l <= 1e-12 max(l) dropped, so fewer
+    than B.shape[0] pairs come back."""
     Ah = A.detach().to("cpu", torch.float64)
-    Linv = _chol_inv(B)
-    At = Linv @ Ah @ Linv.T
+    Bh = 0.5 * (B + B.T).detach().to("cpu", torch.float64)
+    L, info = torch.linalg.cholesky_ex(Bh)
+    if int(info) == 0:
+        Linv = torch.linalg.solve_triangular(L, torch.eye(L.shape[0], dtype=L.dtype), upper=False)
+        At = Linv @ Ah @ Linv.T
+        ev, Y = torch.linalg.eigh(0.5 * (At + At.T))
+        return ev, Linv.T @ Y
+    lb, U = torch.linalg.eigh(Bh)
+    keep = lb > 1e-12 * float(lb.max())
+    X = U[:, keep] / torch.sqrt(lb[keep])[None, :]
+    At = X.T @ Ah @ X
     ev, Y = torch.linalg.eigh(0.5 * (At + At.T))
-    return ev, Linv.T @ Y
+    return ev, X @ Y
 
 
 def _chol_inv(G):
@@ -154,6 +167,9 @@ PRECONDITIONER_FLOOR = 1e-2
 # kinetic preconditioner: CG steps per residual block and the SPD shift c (Ha)
 KINETIC_CG_STEPS = 6
 KINETIC_SHIFT = 0.5
+# residual rows whose (twice) projection out of the basis keeps less than DEPENDENT_REL *
+# eps of their norm are treated as lying in the basis span
+DEPENDENT_REL = 10.0
 # K panel of the split-K Gram products: about NT_PANELS panels, none shorter than NT_MIN_PANEL
 NT_PANELS = 256
 NT_MIN_PANEL = 4096
@@ -228,6 +244,9 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
         raise ValueError(f"hz_update must be `rotate` or `apply`; received `{hz_update}`.")
     m = X0.shape[0]
     reapply = hz_update == "apply" or (hz_update is None and m + k >= HZ_APPLY_MIN)
+    # S is symmetric for the zero-ghost closure and for the legacy operator under its
+    # product weights (the W the callers pass); the one-sided production ends are not
+    symmetric = H.dg.kind == "legacy" or H.closure == "trial_fix"
     if precondition not in (True, False, None, "diagonal", "kinetic"):
         raise ValueError(f"precondition must be `diagonal`, `kinetic` or False; received `{precondition}`.")
     pmode = "kinetic" if precondition is True else (precondition or None)
@@ -261,16 +280,31 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
                                closure=H.closure)
         kdinv64 = 1.0 / (-0.5 * H.dg.kinetic_diagonal() + KINETIC_SHIFT)
 
-    def apply_s(Zb, out, op=H, phase="hamiltonian_apply"):
+    # On monitor grids without projectors the W^(1/2) on the output folds into the
+    # stencil's 1/J and V factors (one pass fewer per apply).
+    fold = H.dg.kind == "monitor" and H.nonlocal_plan is None
+    folded = {}
+
+    def apply_s(Zb, out):
         """out = S Zb = W^(1/2) H W^(-1/2) Zb for a block of rows (in panels of at most
         APPLY_ROWS rows, so the unwhitened temporary stays small)."""
         mark("rotation")
+        dtb = Zb.dtype
+        if fold and dtb not in folded:
+            s_ = sw64.to(dtb)
+            folded.clear()
+            folded[dtb] = (H.dg.invj(dtb) * s_, H.veff[dtb] * s_)
         for r0 in range(0, Zb.shape[0], APPLY_ROWS):
             r1 = min(r0 + APPLY_ROWS, Zb.shape[0])
             xb = (Zb[r0:r1] * isw).view((r1 - r0,) + shape)
-            op.apply(xb, out[r0:r1].view((r1 - r0,) + shape))
-            out[r0:r1].mul_(sw)
-        mark(phase)
+            ob = out[r0:r1].view((r1 - r0,) + shape)
+            if fold:
+                ij, vs = folded[dtb]
+                apply_operator(H.dg, xb, ob, scale=-0.5, closure=H.closure, veff=vs, invj=ij)
+            else:
+                H.apply(xb, ob)
+                out[r0:r1].mul_(sw)
+        mark("hamiltonian_apply")
 
     cg_bufs = {}
 
@@ -442,23 +476,49 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
         GRZ = gemm_nt(R, Z)
         mark("gram")
         R.sub_(torch.matmul(GRZ.to(dt), Z))
-        GR = gemm_nt(R, R)
+        GR = gemm_nt(R, R).cpu()
         mark("gram")
+        # |R_pre|^2 = |R|^2 + |R Z^T|^2 (Z orthonormal).  Rows that lost more than 99 % of
+        # their norm to the projection carry its rounding error: they are projected once
+        # more ("twice is enough").  A row still below DEPENDENT_REL * eps of its norm lies
+        # in span(Z) to rounding and is dropped (zero row, unit pivot) rather than whitened
+        # into noise.
+        d = torch.diagonal(GR).clone()
+        d_pre = d + (GRZ.cpu() ** 2).sum(1)
+        again = d < 1e-4 * d_pre
+        if bool(again.any()):
+            idx = again.nonzero().flatten().to(dev)
+            Rf = R.index_select(0, idx)
+            Rf.sub_(torch.matmul(gemm_nt(Rf, Z).to(dt), Z))
+            R.index_copy_(0, idx, Rf)
+            del Rf
+            GR = gemm_nt(R, R).cpu()
+            d = torch.diagonal(GR).clone()
+            mark("gram")
+        dead = d <= (DEPENDENT_REL * torch.finfo(dt).eps) ** 2 * d_pre
+        if bool(dead.any()):
+            R[dead.to(dev)] = 0
+            GR[dead, :] = 0
+            GR[:, dead] = 0
+            GR[dead, dead] = 1.0
         Linv_r = _chol_inv(GR).to(dev)
         mark("eigh")
         R.copy_(torch.matmul(Linv_r.to(dt), R))
         apply_s(R, HR)
         G23 = gemm_nt(BUF[:m + k], BUF[m:m + 2 * k]).cpu()  # [Z; R] [R; HR]^T: (m + k, 2k)
-        G4 = gemm_nt(R, HZ).cpu()                             # R HZ^T: (k, m)
+        # R HZ^T: (k, m); equal to (Z HR^T)^T up to rounding when S is symmetric
+        G4 = None if symmetric else gemm_nt(R, HZ).cpu()
         mark("gram")
         ZR, ZHR = G23[:m, :k], G23[:m, k:]
         RR, RHR = G23[m:, :k], G23[m:, k:]
         Pz = P.cpu() if P is not None else torch.diag(torch.from_numpy(theta_all))
         Bz = Gzz.cpu() if Gzz is not None else torch.eye(m, dtype=torch.float64)
-        T = _sym(torch.cat([torch.cat([Pz, ZHR], 1), torch.cat([G4, RHR], 1)], 0))
+        T = _sym(torch.cat([torch.cat([Pz, ZHR], 1), torch.cat([ZHR.T if G4 is None else G4, RHR], 1)], 0))
         GB = _sym(torch.cat([torch.cat([Bz, ZR], 1), torch.cat([ZR.T, RR], 1)], 0))
         # generalized Rayleigh-Ritz on [Z; R]: c^T GB c = I, c^T T c = diag(theta)
         thT, c = _host_geigh(T, GB)
+        if c.shape[1] < m:
+            raise RuntimeError("the Rayleigh-Ritz space lost rank below the working-subspace size")
         theta_all = thT[:m].numpy().copy()
         Q = c[:, :m].T.contiguous().to(dev)  # new basis = Q [Z; R]
         mark("eigh")

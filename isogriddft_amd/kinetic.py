@@ -45,12 +45,15 @@ def apply_operator(
     interior_only: bool = False,
     xchunk: int = 0,
     tile: int = 0,
+    invj: torch.Tensor | None = None,
     stream=None,
 ) -> torch.Tensor:
     """out = scale * Lap(x) + veff * x for a block x of shape (k, *shape) or (*shape).
 
     x, out and veff are CUDA tensors of one dtype (float32 / float64); the geometry
-    copy of that dtype is used.  ``closure`` is ignored on legacy grids.
+    copy of that dtype is used.  ``closure`` is ignored on legacy grids.  ``invj``
+    (monitor grids) replaces the 1/J factor of the divergence: with invj = s / J and
+    veff = s V the output comes out scaled point-wise by s at no extra pass.
     """
     if not x.is_cuda:
         raise ValueError("apply_operator takes a CUDA tensor")
@@ -77,10 +80,17 @@ def apply_operator(
     sfx = "f64" if dt == torch.float64 else "f32"
     st = _lib.stream_handle(stream)
     nx, ny, nz = shape
+    if invj is not None:
+        if dg.kind != "monitor":
+            raise ValueError("an invj override applies to monitor grids only")
+        if invj.dtype != dt or invj.numel() != dg.N:
+            raise ValueError("invj must be a field of the block dtype")
+        invj = invj.reshape(-1).contiguous()
     if dg.kind == "monitor":
         cl = _closure_id(closure)
+        ij = dg.invj(dt) if invj is None else invj
         _lib.call(f"ig_monitor_stencil_{sfx}", _lib.ptr(xb), _lib.ptr(ob), nblk, dg.N, nx, ny, nz,
-                  _lib.ptr(dg.get("scoef", dt)), _lib.ptr(dg.invj(dt)), _lib.ptr(veff), float(scale), cl,
+                  _lib.ptr(dg.get("scoef", dt)), _lib.ptr(ij), _lib.ptr(veff), float(scale), cl,
                   int(transpose), int(interior_only), int(xchunk), int(tile), st)
     else:
         _lib.call(f"ig_legacy_stencil_{sfx}", _lib.ptr(xb), _lib.ptr(ob), nblk, dg.N, nx, ny, nz,

@@ -115,6 +115,22 @@ def test_hz_reapply_matches_rotation(precision):
         eigensolver.block_subspace_iteration(H, dg.weights(), X0, k, 1e-6, 4, hz_update="bogus")
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_small_working_subspace(precision):
+    """The smallest working subspace the reference allows (2k + 2) still converges to the
+    dense eigenvalues with the kinetic preconditioner."""
+    g, V = _problem((14, 15, 16))
+    dg = DeviceGrid(g)
+    H = BlockHamiltonian(dg, torch.from_numpy(V).cuda(), closure="trial_fix")
+    k = 4
+    X0 = _start(dg, g, 2 * k + 2)
+    res = eigensolver.block_subspace_iteration(H, dg.weights(), X0, k, 1e-6, 600, precision=precision)
+    ref = _dense_eigs(g, V, k)
+    print(precision, "iters", res.iterations, res.eigenvalues - ref)
+    assert res.converged
+    assert np.max(np.abs(res.eigenvalues - ref)) <= 1e-8
+
+
 def test_orthonormalize_spans_oracle_subspace():
     g, _ = _problem()
     dg = DeviceGrid(g)
