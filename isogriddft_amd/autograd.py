@@ -14,6 +14,10 @@ on the same HIP kernels as the forward path:
                       vectors normalised under the weights w that make T + V self-adjoint
                       (cell volumes on monitor grids, (h s_x)(h s_y)(h s_z) on legacy
                       grids), i.e. dE/dv_ext for E = sum_i g_i eps_i.
+* ``OrbitalDensity``  rho = sum_i f_i psi_i^2 of the lowest-k states of T + V; backward
+                      is the density response: one Sternheimer solve per occupied state
+                      ((H - eps_i) z_i = -P g psi_i / w, projected CG on the complement of
+                      the occupied states), dL/dV = sum_i 2 f_i w psi_i z_i.
 * ``HartreePotential`` v_H = P rho (open-boundary Poisson); backward is the adjoint
                       solve P^T g (transposed stencil in BiCGSTAB, transposed multipole
                       faces and moments), giving dE_H / d rho and the Hartree part of
@@ -95,6 +99,139 @@ class RitzValues(torch.autograd.Function):
 
 def ritz_values(veff, dg, X0, k, tol=1e-8, closure="trial_fix"):
     return RitzValues.apply(veff, dg, X0, k, tol, closure)
+
+
+class OrbitalDensity(torch.autograd.Function):
+    """rho = sum_i f_i psi_i^2 of the lowest-k states of T + V; backward is the density
+    response by Sternheimer solves (recomputed, nothing stored but the orbitals).
+
+    With W the weights that make T + V self-adjoint and psi W-orthonormal, a change dV
+    moves psi_i by sum_{j != i} psi_j <psi_j, dV psi_i>_W / (eps_i - eps_j), so for a loss
+    L = sum_p g_p rho_p
+        dL/dV_p = sum_i 2 f_i w_p psi_i(p) z_i(p),
+        (H - eps_i) z_i = -P (g psi_i / w),   z_i in range(P),
+    P the W-orthogonal projector off the occupied states (f > 0).  Pairs of occupied
+    states contribute (f_i - f_j) / (eps_i - eps_j) <psi_j, g psi_i> w psi_i psi_j, which
+    vanishes for equal occupations and is added explicitly otherwise.  On range(P)
+    H - eps_i is SPD when the occupied states lie below the rest (a gap), so each z_i
+    comes from Jacobi-preconditioned CG in the W inner product, all rows at once with the
+    row-wise HIP kernels and one stencil apply per step.
+    """
+
+    @staticmethod
+    def forward(ctx, veff, dg: DeviceGrid, X0, k, occupations, tol=1e-10, closure="trial_fix", max_iter=400,
+                response_tol=1e-10, response_max_iter=2000):
+        from .eigensolver import block_subspace_iteration
+        from .hamiltonian import BlockHamiltonian
+
+        f = torch.as_tensor(np.asarray(occupations, dtype=np.float64))
+        if f.numel() != k or bool((f < 0).any()):
+            raise ValueError("occupations must be k non-negative numbers")
+        H = BlockHamiltonian(dg, veff.detach().reshape(-1), closure=closure)
+        res = block_subspace_iteration(H, dg.operator_weights(), X0, k, tol, max_iter)
+        if not res.converged:
+            raise RuntimeError("the block eigensolver did not converge inside the autograd forward")
+        psi = res.orbitals.reshape(k, -1).to(torch.float64)
+        fd = f.to(psi.device)
+        ctx.save_for_backward(psi, fd)
+        ctx.eps = np.asarray(res.eigenvalues, dtype=np.float64)
+        ctx.H, ctx.dg, ctx.vshape = H, dg, veff.shape
+        ctx.rtol, ctx.rmax = response_tol, response_max_iter
+        return (fd[:, None] * psi * psi).sum(0).reshape(veff.shape)
+
+    @staticmethod
+    def backward(ctx, g):
+        psi, f = ctx.saved_tensors
+        z, info = sternheimer_solve(ctx.H, psi, ctx.eps, f, g.reshape(-1).to(torch.float64), ctx.rtol, ctx.rmax)
+        if not info["converged"]:
+            raise RuntimeError(f"the Sternheimer solves did not converge: {info}")
+        dg = ctx.dg
+        w = dg.operator_weights()
+        occ = (f > 0).nonzero().flatten()
+        grad = 2.0 * w * (f[occ, None] * psi[occ] * z).sum(0)
+        # occupied pairs with different occupations
+        gf = g.reshape(-1).to(torch.float64)
+        fo = f[occ].cpu().numpy()
+        eo = ctx.eps[occ.cpu().numpy()]
+        for a in range(len(occ)):
+            for b in range(a + 1, len(occ)):
+                if fo[a] == fo[b]:
+                    continue
+                if abs(eo[a] - eo[b]) < 1e-12:
+                    raise RuntimeError("degenerate occupied levels with different occupations")
+                pa, pb = psi[occ[a]], psi[occ[b]]
+                coef = 2.0 * (fo[a] - fo[b]) / (eo[a] - eo[b]) * float((gf * pa * pb).sum())
+                grad = grad + coef * w * pa * pb
+        return grad.reshape(ctx.vshape), None, None, None, None, None, None, None, None, None
+
+
+RESPONSE_SHIFT = 0.5
+
+
+def sternheimer_solve(H, psi, eps, f, g, tol=1e-10, max_iter=2000):
+    """z_i for the occupied rows (f_i > 0): (H - eps_i) z_i = -P (g psi_i / w) on range(P),
+    P = I - Psi_occ Psi_occ^T W.  Projected Jacobi-PCG in the W inner product with
+    D = diag(T) + RESPONSE_SHIFT; r stays in range(P), so (r, P D^-1 r)_W = (r, D^-1 r)_W
+    and the fused row kernels apply, with the direction projected after each update.
+    Returns (z (nocc, N), info)."""
+    from .reductions import cg_direction, cg_update, gram, row_dots
+
+    dg = H.dg
+    N = dg.N
+    shape = dg.shape
+    w = dg.operator_weights()
+    occ = (f > 0).nonzero().flatten()
+    Po = psi[occ].contiguous()
+    nocc = Po.shape[0]
+    e = torch.as_tensor(np.asarray(eps, dtype=np.float64)[occ.cpu().numpy()], device=psi.device)
+    dinv = 1.0 / (-0.5 * dg.kinetic_diagonal() + RESPONSE_SHIFT)
+
+    def project(y):
+        C = gram(y, Po, w)                       # (rows, nocc) = y W Po^T
+        y.sub_(C @ Po)
+
+    def apply_shifted(pv, out):
+        H.apply(pv.view((nocc,) + shape), out.view((nocc,) + shape))
+        out.addcmul_(e[:, None], pv, value=-1.0)
+        project(out)
+
+    u = (g[None, :] * Po) / w[None, :]
+    # convergence is measured against the unprojected right-hand side, so a row whose
+    # projection leaves only rounding (e.g. g proportional to w) counts as solved
+    thresh = tol * tol * row_dots(u, u * dinv, w)
+    b = -u
+    del u
+    project(b)
+    x = torch.zeros_like(b)
+    r = b
+    pv = r * dinv
+    project(pv)
+    q = torch.empty_like(b)
+    rz = row_dots(r, r * dinv, w)
+    it = 0
+    converged = bool((rz <= thresh).all())
+    while not converged and it < max_iter:
+        active = rz > thresh
+        apply_shifted(pv, q)
+        pq = row_dots(pv, q, w)
+        if bool((pq[active] <= 0).any()):
+            return x, {"converged": False, "iterations": it, "reason": "operator not positive on the complement"}
+        alpha = torch.where(active & (pq > 0), rz / torch.where(pq > 0, pq, 1.0), 0.0)
+        rz_new = cg_update(x, r, pv, q, dinv, w, alpha)
+        beta = torch.where(active & (rz > 0), rz_new / torch.where(rz > 0, rz, 1.0), 0.0)
+        cg_direction(pv, r, dinv, beta)
+        project(pv)
+        rz = rz_new
+        it += 1
+        if it % 10 == 0 or it == max_iter:
+            converged = bool((rz <= thresh).all())
+    rel = torch.sqrt(rz / torch.where(thresh > 0, thresh / (tol * tol), 1.0))
+    return x, {"converged": bool((rz <= thresh).all()), "iterations": it, "relative_residual": float(rel.max())}
+
+
+def orbital_density(veff, dg, X0, k, occupations, tol=1e-10, closure="trial_fix", response_tol=1e-10):
+    """sum_i f_i psi_i^2 of the lowest-k states of T + veff, differentiable in veff."""
+    return OrbitalDensity.apply(veff, dg, X0, k, occupations, tol, closure, 400, response_tol, 2000)
 
 
 class HartreePotential(torch.autograd.Function):

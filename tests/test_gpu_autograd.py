@@ -92,3 +92,37 @@ def test_forces_match_finite_differences():
             fd = -(ep - em) / (2 * h)
             print(a, ax, F[a, ax], fd)
             assert abs(F[a, ax] - fd) <= 1e-5 * max(abs(fd), 1.0), (a, ax, F[a, ax], fd)
+
+
+@pytest.mark.parametrize("occupations", [(2.0, 2.0, 0.0, 0.0), (2.0, 1.0, 1.0, 0.0)])
+def test_density_response_matches_finite_differences(occupations):
+    """d/dv of L = sum_p g_p rho_p[v] (rho = sum_i f_i psi_i^2 of the lowest states of
+    T + v) from the Sternheimer adjoint, against central differences of re-solved
+    eigenproblems; the second case has unequal occupations (occupied-pair terms)."""
+    g = configs.build_synthetic_geometry(shape=(14, 15, 16))
+    dg = DeviceGrid(g)
+    v0 = -2.0 * np.exp(-0.5 * ((g.x_points - 0.3) ** 2 + 1.3 * g.y_points ** 2 + 0.8 * (g.z_points + 0.2) ** 2)) \
+        - 1.2 * np.exp(-0.7 * ((g.x_points + 0.9) ** 2 + (g.y_points - 0.6) ** 2 + (g.z_points - 0.8) ** 2))
+    dv = 0.5 * np.exp(-0.3 * ((g.x_points - 0.5) ** 2 + g.y_points ** 2 + (g.z_points + 0.4) ** 2))
+    gl = np.cos(0.4 * g.x_points) * np.exp(-0.1 * (g.y_points ** 2 + g.z_points ** 2)) + 0.3 * g.z_points
+    k = 4
+    X0 = eigensolver.default_guess_block(g, [("H", (0.0, 0.0, -0.7)), ("H", (0.0, 0.0, 0.7))],
+                                         eigensolver.reference_block_size(k), dg)
+    gt = torch.from_numpy(gl.reshape(-1)).cuda()
+    v = torch.from_numpy(v0.reshape(-1)).cuda().requires_grad_(True)
+    rho = autograd.orbital_density(v, dg, X0, k, occupations, tol=1e-10)
+    (grad,) = torch.autograd.grad((gt * rho).sum(), v)
+    analytic = float((grad.cpu().numpy() * dv.reshape(-1)).sum())
+    h = 1e-4
+
+    def loss(vv):
+        r = autograd.orbital_density(torch.from_numpy(vv.reshape(-1)).cuda(), dg, X0, k, occupations, tol=1e-10)
+        return float((gt * r).sum())
+
+    fd = (loss(v0 + h * dv) - loss(v0 - h * dv)) / (2 * h)
+    print(occupations, "dL/dv . dv", analytic, fd)
+    assert abs(analytic - fd) <= 1e-5 * abs(fd)
+    # the charge is conserved to first order: d(sum w rho)/dv = 0
+    w = dg.operator_weights()
+    (gq,) = torch.autograd.grad((w * autograd.orbital_density(v, dg, X0, k, occupations, tol=1e-10)).sum(), v)
+    assert float(gq.abs().max()) <= 1e-8 * float(grad.abs().max())
