@@ -100,6 +100,9 @@ class OpenBoundaryPoissonResult:
 # ------------------------------------------------------------------ monomial algebra
 
 
+# smallest relative residual asked of one FP32 inner solve
+MIXED_INNER_FLOOR = 1e-6
+
 def monomial_exponents(order: int):
     """Exponent triples of the raw-moment table: degree by degree, (px, py, pz)
     descending lexicographically (the layout the moment kernels write)."""
@@ -568,8 +571,10 @@ def solve_poisson_device(dg, rho, *, multipole_order=2, tolerance=1e-8, max_iter
                                  check_every=check_every, stream=stream)
         method = "hip_bicgstab_fp64" + ("_monitor" if dg.kind == "monitor" else "")
     else:
-        # FP32 BiCGSTAB corrections of the FP64 residual (defect correction)
-        inner = inner_tolerance if inner_tolerance is not None else max(min(1e-4, tolerance * 1e3), 1e-6)
+        # FP32 BiCGSTAB corrections of the FP64 residual (defect correction).  Each inner
+        # solve restarts its Krylov space, so it is asked for the whole remaining
+        # reduction (at most what FP32 attains, MIXED_INNER_FLOOR): the first one goes to
+        # ~1e-6, the next only closes the gap to the tolerance.
         r64 = rhs.clone()
         tmp = torch.empty_like(rhs)
         d32 = torch.zeros(rho.numel(), dtype=torch.float32, device=rho.device)
@@ -583,6 +588,8 @@ def solve_poisson_device(dg, rho, *, multipole_order=2, tolerance=1e-8, max_iter
             outer += 1
             scale = float(torch.linalg.vector_norm(r64).item())
             r32 = (r64 / scale).float()
+            inner = inner_tolerance if inner_tolerance is not None else \
+                max(0.5 * tolerance * rn / scale, MIXED_INNER_FLOOR)
             k, _, _ = bicgstab(dg, r32, d32, dinv=dinv32, rtol=inner, maxiter=max_iterations - it,
                                check_every=check_every, stream=stream)
             it += k
