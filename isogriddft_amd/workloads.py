@@ -1,0 +1,159 @@
+"""Molecular fixed-potential workloads on atom-centred monitor grids, end to end on the GPU.
+
+``python -m isogriddft_amd.workloads benzene [--n 224] [--k 48]`` runs:
+
+  grid       the harmonic-map monitor generator for the molecule (grid/monitor_builder.py
+             defaults for H/C/N/O), n^3 points over +-half_extent Bohr
+  density    a superposition of normalised atomic Gaussians carrying the GTH valence
+             charges (a frozen start density; the reference starts its SCF from a
+             superposition as well), spin-unpolarised
+  potential  V_loc (GTH-PADE) + v_H (open-boundary Poisson, multipole order 4) + v_xc
+             (LSDA, lda,vwn) and the separable GTH projectors
+  eigensolve the lowest k states of that fixed potential, in FP64 and in FP32 + FP64
+             refinement (kinetic preconditioner, working subspace `block`)
+  forces     Hellmann-Feynman forces -dE_ion/dR at the FP64 orbitals (ionic-energy adjoint)
+
+and prints one JSON line with the timings, iteration counts and the mixed-vs-FP64
+eigenvalue difference.  Benzene: D6h, C-C 2.637 Bohr, C-H 2.048 Bohr, 30 valence
+electrons, k = 48 (15 occupied + 33 buffer states), +-14 Bohr.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+import torch
+
+from . import autograd, configs, eigensolver, monitor_grid, poisson, pseudo, xc
+from .device import DeviceGrid
+from .hamiltonian import BlockHamiltonian
+from .reductions import wdot
+
+# Gaussian exponents (Bohr^-2) of the atomic valence densities of the start density
+START_EXPONENTS = {"H": 1.0, "C": 0.55, "N": 0.65, "O": 0.75}
+
+MOLECULES = {"benzene": (configs.BENZENE, 14.0, 48, 15), "h2o": (configs.H2O, 10.0, 8, 4)}
+
+
+def _log(msg):
+    print(f"[workload] {msg}", file=sys.stderr, flush=True)
+
+
+def start_density(dg, atoms):
+    """sum_a Z_a (alpha/pi)^(3/2) exp(-alpha r_a^2), renormalised on the grid to sum Z_a."""
+    xyz = dg.xyz()
+    rho = torch.zeros(dg.N, dtype=torch.float64, device=dg.device)
+    ztot = 0.0
+    for el, pos in atoms:
+        z = pseudo.GTH_PADE[el].ionic_charge
+        a = START_EXPONENTS[el]
+        r2 = (xyz[0] - pos[0]) ** 2 + (xyz[1] - pos[1]) ** 2 + (xyz[2] - pos[2]) ** 2
+        rho += z * (a / np.pi) ** 1.5 * torch.exp(-a * r2)
+        ztot += z
+    return rho * (ztot / wdot(dg, rho, None))
+
+
+def run(name="benzene", n=224, k=None, block=None, tol=3e-4, multipole_order=4, grid_device="cuda"):
+    mol, half, k_default, nocc = MOLECULES[name]
+    k = k or k_default
+    atoms = pseudo.case_atoms(mol)
+    out = {"workload": f"{mol.name} fixed-potential eigensolve on a {n}^3 monitor grid (+-{half} Bohr)",
+           "k": k, "occupied": nocc}
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    geom = monitor_grid.build_monitor_grid_for_case(
+        mol, shape=(n, n, n), box_half_extents=(half, half, half),
+        element_parameters=monitor_grid.build_default_near_core_element_parameters(mol), device=grid_device)
+    torch.cuda.synchronize()
+    out["grid_s"] = round(time.perf_counter() - t0, 3)
+    out["grid_min_jacobian"] = float(np.min(geom.jacobian))
+    _log(f"grid {n}^3 in {out['grid_s']} s")
+    dg = DeviceGrid(geom)
+
+    t0 = time.perf_counter()
+    rho = start_density(dg, atoms)
+    vloc = pseudo.local_potential_device(dg, atoms)
+    half_rho = 0.5 * rho
+    _, vxc, _ = xc.lsda_device(half_rho, half_rho)
+    torch.cuda.synchronize()
+    out["potential_setup_s"] = round(time.perf_counter() - t0, 3)
+    pots = {}
+    for prec in ("fp64", "mixed"):
+        kw = dict(multipole_order=multipole_order, tolerance=1e-8, precision=prec, max_iterations=4000)
+        poisson.solve_poisson_device(dg, rho, **kw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sol = poisson.solve_poisson_device(dg, rho, **kw)
+        torch.cuda.synchronize()
+        out[f"poisson_{prec}_s"] = round(time.perf_counter() - t0, 4)
+        out[f"poisson_{prec}_iterations"] = sol["iterations"]
+        pots[prec] = sol["potential"].reshape(-1)
+    out["poisson_mixed_vs_fp64_max_rel"] = float((pots["mixed"] - pots["fp64"]).abs().max()
+                                                 / pots["fp64"].abs().max())
+    _log(f"poisson fp64 {out['poisson_fp64_s']} s, mixed {out['poisson_mixed_s']} s")
+    veff = vloc + pots["fp64"] + vxc.reshape(-1)
+    t0 = time.perf_counter()
+    plan = pseudo.NonlocalPlan(dg, atoms)
+    out["projector_setup_s"] = round(time.perf_counter() - t0, 3)
+    H = BlockHamiltonian(dg, veff, closure="trial_fix", nonlocal_plan=plan)
+    m = block or 4 * k + 8
+    X0 = eigensolver.default_guess_block(geom, atoms, m, dg)
+    _log(f"starting block of {m} orbitals built")
+    ev = {}
+    res64 = None
+    for prec in ("fp64", "mixed"):
+        eigensolver.block_subspace_iteration(H, dg.weights(), X0, k, tol, 3, precision=prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = eigensolver.block_subspace_iteration(H, dg.weights(), X0, k, tol, 600, precision=prec, profile=True)
+        torch.cuda.synchronize()
+        out[f"eig_{prec}_s"] = round(time.perf_counter() - t0, 3)
+        out[f"eig_{prec}_iterations"] = res.iterations
+        out[f"eig_{prec}_converged"] = bool(res.converged)
+        out[f"eig_{prec}_phase_s"] = {key: round(v, 3) for key, v in res.timings.items()}
+        ev[prec] = np.asarray(res.eigenvalues)
+        if prec == "fp64":
+            res64 = res
+        else:
+            out["eig_fp32_iterations"] = res.fp32_iterations
+        _log(f"eigensolve {prec}: {out[f'eig_{prec}_s']} s, {res.iterations} iterations")
+    out["eigenvalues_fp64_Ha"] = [round(float(e), 10) for e in ev["fp64"]]
+    out["eig_mixed_vs_fp64_max_abs_Ha"] = float(np.max(np.abs(ev["mixed"] - ev["fp64"])))
+    out["eigensolve_plus_poisson_fp64_s"] = round(out["eig_fp64_s"] + out["poisson_fp64_s"], 3)
+    out["eigensolve_plus_poisson_mixed_s"] = round(out["eig_mixed_s"] + out["poisson_mixed_s"], 3)
+
+    occupations = [2.0] * nocc
+    orbs = res64.orbitals.reshape(k, -1)[:nocc].to(torch.float64)
+    rho_orb = (2.0 * orbs * orbs).sum(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    F = autograd.atomic_forces(dg, [el for el, _ in atoms], [p for _, p in atoms], rho_orb, orbs, occupations)
+    torch.cuda.synchronize()
+    out["forces_s"] = round(time.perf_counter() - t0, 3)
+    out["forces_Ha_per_Bohr"] = np.round(F, 8).tolist()
+    out["forces_net_Ha_per_Bohr"] = np.round(F.sum(0), 10).tolist()
+    _log(f"forces in {out['forces_s']} s")
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("molecule", choices=sorted(MOLECULES))
+    ap.add_argument("--n", type=int, default=224, help="grid points per axis")
+    ap.add_argument("--k", type=int, default=None, help="states (default: the molecule's k)")
+    ap.add_argument("--block", type=int, default=None, help="working subspace (default 4k + 8)")
+    ap.add_argument("--tol", type=float, default=3e-4, help="eigensolver residual tolerance")
+    args = ap.parse_args(argv)
+    if not torch.cuda.is_available():
+        print("the workloads run on the GPU only", file=sys.stderr)
+        sys.exit(2)
+    print(json.dumps(run(args.molecule, args.n, args.k, args.block, args.tol)), flush=True)
+
+
+if __name__ == "__main__":
+    main()
