@@ -126,3 +126,45 @@ def test_density_response_matches_finite_differences(occupations):
     w = dg.operator_weights()
     (gq,) = torch.autograd.grad((w * autograd.orbital_density(v, dg, X0, k, occupations, tol=1e-10)).sum(), v)
     assert float(gq.abs().max()) <= 1e-8 * float(grad.abs().max())
+
+
+def test_composite_energy_gradient_16_random_directions():
+    """L(v) = sum_i f_i eps_i[v] + E_H[rho[v]] through three adjoints chained by autograd
+    (Ritz values, the Sternheimer density response, the adjoint Poisson solve), against
+    central differences along 16 seeded random smooth directions."""
+    g = configs.build_synthetic_geometry(shape=(14, 15, 16))
+    dg = DeviceGrid(g)
+    v0 = -2.0 * np.exp(-0.5 * ((g.x_points - 0.3) ** 2 + 1.3 * g.y_points ** 2 + 0.8 * (g.z_points + 0.2) ** 2)) \
+        - 1.2 * np.exp(-0.7 * ((g.x_points + 0.9) ** 2 + (g.y_points - 0.6) ** 2 + (g.z_points - 0.8) ** 2))
+    k, occ = 4, (2.0, 2.0, 0.0, 0.0)
+    f = torch.tensor(occ, dtype=torch.float64, device="cuda")
+    X0 = eigensolver.default_guess_block(g, [("H", (0.0, 0.0, -0.7)), ("H", (0.0, 0.0, 0.7))],
+                                         eigensolver.reference_block_size(k), dg)
+
+    def L(v):
+        eps = autograd.ritz_values(v, dg, X0, k, tol=1e-10)
+        rho = autograd.orbital_density(v, dg, X0, k, occ, tol=1e-10)
+        return (f * eps).sum() + autograd.hartree_energy(rho, dg, tol=1e-11)
+
+    v = torch.from_numpy(v0.reshape(-1)).cuda().requires_grad_(True)
+    (grad,) = torch.autograd.grad(L(v), v)
+    grad = grad.cpu().numpy()
+    rng = np.random.default_rng(2024)
+    h = 1e-4
+    worst = 0.0
+    for j in range(16):
+        dv = np.zeros(g.x_points.shape)
+        for _ in range(3):
+            c = rng.uniform(-2.0, 2.0, 3)
+            a = rng.uniform(0.2, 1.0)
+            dv += rng.normal() * np.exp(-a * ((g.x_points - c[0]) ** 2 + (g.y_points - c[1]) ** 2
+                                               + (g.z_points - c[2]) ** 2))
+        analytic = float((grad * dv.reshape(-1)).sum())
+        lp = L(torch.from_numpy((v0 + h * dv).reshape(-1)).cuda()).item()
+        lm = L(torch.from_numpy((v0 - h * dv).reshape(-1)).cuda()).item()
+        fd = (lp - lm) / (2 * h)
+        rel = abs(analytic - fd) / max(abs(fd), 1e-8)
+        worst = max(worst, rel)
+        print(j, analytic, fd, rel)
+        assert abs(analytic - fd) <= 1e-5 * max(abs(fd), 1e-3), (j, analytic, fd)
+    print("worst relative difference over 16 directions", worst)
