@@ -883,7 +883,12 @@ int launch_monitor(const MonParams<T>& p, int closure, int transpose, int tile, 
   if (transpose) return closure ? launch_monitor_t<T, 8, 32, 1, true>(p, s) : launch_monitor_t<T, 8, 32, 0, true>(p, s);
   if (tile < 0) return IG_EINVAL;
   if (tile >= 7000000) return closure ? IG_EUNSUPPORTED : launch_monitor_zv<T>(p, tile - 7000000, s);
-  if (tile == 0 && closure == 0) {
+  // FP32 blocks whose geometry stream (8 fields, 32 B a point) outgrows the 256 MB
+  // Infinity Cache: four orbitals per workgroup (general kernel) halve the geometry
+  // re-reads and beat the z-vectorised kernel's two (224^3, k = 48: 2248 against
+  // 2070 GB/s; at 96^3, where the geometry stays cached, 2368 against 2629).
+  const bool fp32_big = sizeof(T) == 4 && p.nblk >= 4 && (long long)p.nx * p.ny * p.nz * 32 > (128LL << 20);
+  if (tile == 0 && closure == 0 && !fp32_big) {
     const int st = launch_monitor_zv<T>(p, 0, s);
     if (st != IG_EUNSUPPORTED) return st;
   }

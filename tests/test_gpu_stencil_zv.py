@@ -84,3 +84,18 @@ def test_zv_rejects_unsupported_shapes():
     out = kinetic.apply_operator(dg, x, scale=-0.5, closure="trial_fix")
     ref = okin.monitor_kinetic(x[0].cpu().numpy(), g, "trial_fix")
     assert np.max(np.abs(out[0].cpu().numpy() - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_large_fp32_default_dispatch_is_bitwise_equal():
+    """FP32 blocks whose geometry outgrows the Infinity Cache (here 168^3 > 4.2 M points)
+    go by default to the general kernel with four orbitals per workgroup; the result must
+    equal the z-vectorised kernel's bit for bit."""
+    shape = (168, 168, 168)
+    g = configs.build_synthetic_geometry(shape=shape)
+    dg = DeviceGrid(g)
+    rng = np.random.default_rng(3)
+    x = torch.from_numpy(rng.standard_normal((4,) + shape)).float().cuda()
+    vt = torch.from_numpy(0.3 * np.sin(g.x_points) * np.cos(g.z_points)).float().cuda().reshape(-1)
+    default = kinetic.apply_operator(dg, x, scale=-0.5, closure="trial_fix", veff=vt)
+    zv = kinetic.apply_operator(dg, x, scale=-0.5, closure="trial_fix", veff=vt, tile=ZV + 2)
+    assert torch.equal(default, zv)
