@@ -21,3 +21,14 @@ def test_h2o_workload_small():
     # are opposite, and no atom feels a y force (to the eigensolver tolerance)
     assert abs(F[1, 0] + F[2, 0]) <= 1e-4 * max(abs(F[1, 0]), 1e-3)
     assert np.max(np.abs(F[:, 1])) <= 1e-4 * np.max(np.abs(F))
+
+
+def test_h2_legacy_workload():
+    """The reference's CPU-runnable case (legacy 49 x 49 x 61, lda,pw92, k = 2): converges,
+    a bound bonding state, a physical Hartree energy of the singlet density."""
+    out = workloads.run_h2_legacy()
+    print(out)
+    assert out["eig_converged"]
+    e = out["eigenvalues_Ha"]
+    assert -1.0 < e[0] < 0.0 and e[0] < e[1]
+    assert 1.0 < out["hartree_energy_Ha"] < 2.5
