@@ -54,6 +54,8 @@ from .kinetic import apply_operator
 from .reductions import cg_direction, cg_update, gram, row_dots
 
 FP32_RESIDUAL_FLOOR = 5e-5
+# an FP32 plateau promotes early only within this factor of the FP32 floor
+FP32_STALL_CEILING = 100.0
 
 
 # ------------------------------------------------------------------ weighted block algebra
@@ -433,11 +435,17 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
         res.subspace_dimensions.append(m)
         ritz_k = P[:k, :k].cpu().numpy() if P is not None else np.diag(th)
         target = max(tol, fp32_floor) if phase32 else tol
-        # FP32 stops helping once the residual stagnates: the best of the last 60
-        # iterations is not 5 % below the best before them
+        # FP32 stops helping once the residual stagnates near its rounding floor: the best
+        # of the last 60 iterations is not 5 % below the best before them while the
+        # residual is within FP32_STALL_CEILING of the floor.  A plateau far above the
+        # floor (the slowest of many states, in exact arithmetic too) only promotes after
+        # 150 iterations without progress.
         hist = res.residual_history
-        stalled = phase32 and len(hist) > 60 and \
-            min(r.max() for r in hist[-60:]) >= 0.95 * min(r.max() for r in hist[:-60])
+
+        def plateau(n):
+            return len(hist) > n and min(r.max() for r in hist[-n:]) >= 0.95 * min(r.max() for r in hist[:-n])
+
+        stalled = phase32 and ((rn.max() <= FP32_STALL_CEILING * fp32_floor and plateau(60)) or plateau(150))
         if rn.max() <= target or stalled:
             if phase32:
                 # refinement: promote the basis to FP64 and rebuild S Z in FP64
