@@ -199,6 +199,8 @@ def _gemm_nt_split(A, B):
     if n0 < N:
         G += torch.matmul(A[:, n0:], B[:, n0:].T).to(torch.float64)
     return G
+
+
 # rows per H apply panel and columns per in-place rotation panel
 APPLY_ROWS = 64
 ROTATION_PANEL = 1 << 20
@@ -227,14 +229,14 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
     polynomial approximation of the inverse kinetic energy that damps the smooth error as
     well as the rough, at one kinetic block apply of k rows per step.  "diagonal" scales by
     (diag(H) - theta_i)^-1 (Davidson; diag(S) = diag(H) since S is a diagonal similarity
-    of H).  False / None: no preconditioner (the reference's iteration).  On stretched grids the kinetic diagonal varies over orders of
-    magnitude and this removes most of the stiffness; the Ritz pairs it converges to are
-    the same.
+    of H).  False / None: no preconditioner (the reference's iteration).  Every choice
+    converges to the same Ritz pairs; only the iteration count changes.
 
     inplace_rotation: rotate the basis in column panels inside one buffer instead of
     alternating two buffers (half the memory, one extra panel copy per rotation).  None
-    (default) takes the two-buffer form when both fit in the free device memory with room
-    to spare, which holds up to about 120 GB of FP64 buffer on one MI355X.
+    (default) takes the two-buffer form when the second buffer fits in 45 % of the free
+    device memory.  Mixed precision always alternates the two FP32 halves of one FP64-sized
+    storage.
 
     hz_update: how S Z follows the new basis: "rotate" (HZ' = Q [HR; HZ], 2 m (m + k) N
     flop) or "apply" (HZ' = S Z', a bandwidth-bound block apply).  None (default) applies
