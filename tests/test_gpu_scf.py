@@ -75,3 +75,54 @@ def test_h2_legacy_singlet_scf_matches_reference():
     _, r5, e5, _ = history[4]
     assert abs(r5 - 1.750856831863596e-3) < 1e-5
     assert abs(e5 - E_TOTAL_REF) < 2e-6
+
+
+def _antibonding(g):
+    a, b = ogrid.H2_CASE.atoms
+    fa = np.exp(-0.8 * ((g.x_points - a.position[0]) ** 2 + (g.y_points - a.position[1]) ** 2
+                        + (g.z_points - a.position[2]) ** 2))
+    fb = np.exp(-0.8 * ((g.x_points - b.position[0]) ** 2 + (g.y_points - b.position[1]) ** 2
+                        + (g.z_points - b.position[2]) ** 2))
+    f = fa - fb
+    return f / np.sqrt(np.sum(f * f * g.cell_volumes))
+
+
+def test_h2_legacy_triplet_scf_matches_reference():
+    """The triplet route of the same recorded run (audit/baselines.py:1966-1977): both
+    electrons spin-up in the two lowest states, converged by the reference after 6
+    iterations (residual 2.248e-3) at E_total = -0.7660687190478731 Ha, lowest eigenvalue
+    -0.6255731151927522 Ha.  Measured: iteration 6 at -0.76606849 Ha, converged
+    -0.76607043 Ha (19 iterations, 1.7e-6 Ha below the reference's stopping point)."""
+    g = ogrid.build_legacy_geometry()
+    occ = energy.SpinOccupations(2.0, 0.0, (1.0, 1.0), ())
+    guess = np.stack([_bonding(g), _antibonding(g)])
+    rho = _renorm(guess[0] ** 2 + guess[1] ** 2, 2.0, g)
+    zero = np.zeros_like(rho)
+    empty = np.zeros((0,) + rho.shape)
+    history = []
+    for it in range(1, 81):
+        sol = eigensolver.solve_fixed_potential_eigenproblem(g, rho, zero, "up", 2, initial_guess_orbitals=guess,
+                                                             tolerance=1e-8)
+        assert sol.converged
+        orbs = np.asarray(sol.orbitals)
+        guess = orbs
+        rho_out = _renorm(orbs[0] ** 2 + orbs[1] ** 2, 2.0, g)
+        residual = float(kinetic.weighted_l2_norm(rho_out - rho, g))
+        e = energy.evaluate_single_point_energy(rho_out, zero, orbs, empty, occ, g)
+        history.append((it, residual, e.total, float(sol.eigenvalues[0])))
+        rho = _renorm(0.4 * rho + 0.6 * rho_out, 2.0, g)
+        if residual < 1e-7:
+            break
+    for row in history:
+        print(row)
+    _, residual, e_total, eps1 = history[-1]
+    print("triplet E_total", e_total, "eps1", eps1, e)
+    assert residual < 1e-7
+    assert abs(e_total - (-0.7660687190478731)) < 1e-5
+    assert abs(eps1 - (-0.6255731151927522)) < 2e-3
+    # at the reference's last iteration (6) the energy agrees to 2.3e-7 Ha; the residual
+    # (2.54e-3 against 2.25e-3) follows the two-state subspace, which the reference's
+    # eigensolver resolved only to its tolerance 5e-3
+    _, r6, e6, _ = history[5]
+    assert abs(r6 - 2.2480823759300898e-3) < 0.2 * 2.2480823759300898e-3
+    assert abs(e6 - (-0.7660687190478731)) < 1e-6
