@@ -197,8 +197,11 @@ def evaluate_static_local_single_point_energy(rho_up, rho_down, orbitals_up, orb
                                               patch_parameters=None, kinetic_version="production",
                                               hartree_backend="python", use_jax_hartree_cached_operator=False,
                                               jax_hartree_cg_impl="baseline", jax_hartree_cg_preconditioner="none",
-                                              jax_hartree_line_preconditioner_impl="baseline"):
-    """scf/driver.py:2668-2810 on the GPU: T_s + E_loc + E_H + E_xc + E_II (E_nl = 0)."""
+                                              jax_hartree_line_preconditioner_impl="baseline",
+                                              monitor_boundary_construction_mode="corrected_moments"):
+    """scf/driver.py:2668-2810 on the GPU: T_s + E_loc + E_H + E_xc + E_II (E_nl = 0).
+    ``monitor_boundary_construction_mode`` (an addition) selects the Hartree faces as in
+    the Poisson surfaces."""
     from . import poisson, xc
     from .reductions import wdot
 
@@ -220,7 +223,7 @@ def evaluate_static_local_single_point_energy(rho_up, rho_down, orbitals_up, orb
         params = default_monitor_patch_parameters() if patch_parameters is None else patch_parameters
         plan = pseudo.patch_plan(dg, case, grid_geometry, params)
         eloc += float(sum(c[3] for c in plan.corrections(dg, rho)))
-    sol = poisson.solve_poisson_device(dg, rho.clamp_min(0.0), tolerance=1e-8)
+    sol = poisson.solve_poisson_device(dg, rho.clamp_min(0.0), tolerance=1e-8, mode=monitor_boundary_construction_mode)
     eh = 0.5 * wdot(dg, rho, sol["potential"].reshape(-1))
     eps, _, _ = xc.lsda_device(ru.view(dg.shape), rd.view(dg.shape))
     exc = wdot(dg, rho, eps.reshape(-1))

@@ -46,7 +46,9 @@ from .reductions import dots_into, workspace
 FOUR_PI = 4.0 * math.pi
 _NEGATIVE_DENSITY_TOLERANCE = 1.0e-14
 _QUANTILE = 0.9  # open_boundary.py:36
-_MODES = ("corrected_moments", "legacy_split")
+# "nodal" (faces from the nodal raw moments alone) is the monitor-grid construction of
+# earlier reference versions, with which the reference's recorded SCF audits were made
+_MODES = ("corrected_moments", "legacy_split", "nodal")
 _SOLVERS = ("auto", "scipy_bicgstab", "jacobi", "bicgstab")
 _JACOBI_DAMPING = 0.8  # open_boundary.py:34
 _JACOBI_CHECK_INTERVAL = 25  # open_boundary.py:35
@@ -393,7 +395,7 @@ def multipole_boundary_device(dg, rho, order=2, mode="corrected_moments", stream
     base = _moments_with_frac(dg, rho, order, None, stream)
     moments = base
     split = mode == "legacy_split" and dg.kind == "monitor"
-    if dg.kind == "monitor":
+    if dg.kind == "monitor" and mode != "nodal":
         moments = base + _corrected_delta(dg, poisson_data(dg), rho, order, stream)
     # legacy_split (open_boundary.py:923-934): faces from the nodal moments plus the split
     # correction; the reported moments stay the corrected ones.
@@ -638,7 +640,7 @@ def moments_adjoint_device(dg, q, order, mode="corrected_moments", out=None, str
 
     qp = q.ctypes.data_as(ctypes.c_void_p)
     cptr, keep = _center_arg(dg.reference_center)
-    corrected = dg.kind == "monitor" and pdata.ncells > 0
+    corrected = dg.kind == "monitor" and pdata.ncells > 0 and mode != "nodal"
     _lib.call("ig_moments_adjoint_f64", _lib.ptr(dg.weights()), _lib.ptr(pdata.frac if corrected else None),
               _lib.ptr(dg.xyz()), dg.N, cptr, order, qp, _lib.ptr(out), 0, _lib.stream_handle(stream))
     if corrected:

@@ -126,3 +126,53 @@ def test_h2_legacy_triplet_scf_matches_reference():
     _, r6, e6, _ = history[5]
     assert abs(r6 - 2.2480823759300898e-3) < 0.2 * 2.2480823759300898e-3
     assert abs(e6 - (-0.7660687190478731)) < 1e-6
+
+
+def test_h2_monitor_patch_triplet_scf_matches_reference():
+    """The monitor-grid route of the same recorded run (audit/baselines.py:1990-2001 via
+    audit/h2_monitor_grid_scf_dry_run_audit.py:28-33): the 67 x 67 x 81 development grid
+    (here from the GPU generator), near-core patch (0.75, 25^3, 1.30, 8 neighbours),
+    trial_fix kinetic, static-local chain, triplet, linear mixing 0.20.  The reference
+    stopped after 18 iterations at residual 4.552787297010315e-3, E_total =
+    -1.2214418066604806 Ha and lowest eigenvalue -0.4168423341628571 Ha.
+
+    Those numbers were made with the nodal multipole faces of earlier reference versions.
+    With the corrected-moment faces of the current reference the same loop retraces the
+    residuals but sits 0.6 mHa lower in energy and eigenvalue: a near-constant offset of
+    v_H.  So the loop runs with monitor_boundary_construction_mode="nodal" (faces from the
+    nodal moments alone).  Measured: at iteration 18 the residual is 4.5476e-3, E_total
+    -1.22144894 Ha (7e-6 from the record) and eps_1 -0.41683993 Ha (2.4e-6)."""
+    from isogriddft_amd import monitor_grid, pseudo
+
+    g = monitor_grid.build_monitor_grid_for_case(
+        ogrid.H2_CASE, shape=(67, 67, 81), box_half_extents=(8.0, 8.0, 10.0),
+        element_parameters=monitor_grid.build_h2_local_patch_development_element_parameters())
+    params = pseudo.LocalPotentialPatchParameters(patch_radius_scale=0.75, patch_grid_shape=(25, 25, 25),
+                                                  correction_strength=1.30, interpolation_neighbors=8)
+    occ = energy.SpinOccupations(2.0, 0.0, (1.0, 1.0), ())
+    guess = np.stack([_bonding(g), _antibonding(g)])
+    rho = _renorm(guess[0] ** 2 + guess[1] ** 2, 2.0, g)
+    zero = np.zeros_like(rho)
+    empty = np.zeros((0,) + rho.shape)
+    history = []
+    for it in range(1, 19):
+        sol = eigensolver.solve_fixed_potential_static_local_eigenproblem(
+            g, rho, zero, "up", 2, initial_guess_orbitals=guess, tolerance=1e-8, use_monitor_patch=True,
+            patch_parameters=params, kinetic_version="trial_fix", monitor_boundary_construction_mode="nodal")
+        assert sol.converged
+        orbs = np.asarray(sol.orbitals)
+        guess = orbs
+        rho_out = _renorm(orbs[0] ** 2 + orbs[1] ** 2, 2.0, g)
+        residual = float(kinetic.weighted_l2_norm(rho_out - rho, g))
+        e = energy.evaluate_static_local_single_point_energy(
+            rho_out, zero, orbs, empty, occ, g, use_monitor_patch=True, patch_parameters=params,
+            kinetic_version="trial_fix", monitor_boundary_construction_mode="nodal")
+        history.append((it, residual, e.total, float(sol.eigenvalues[0])))
+        rho = _renorm(0.8 * rho + 0.2 * rho_out, 2.0, g)
+    for row in history:
+        print(row)
+    _, r18, e18, eps18 = history[-1]
+    print("monitor triplet at iteration 18", r18, e18, eps18, e)
+    assert abs(r18 - 4.552787297010315e-3) < 0.01 * 4.552787297010315e-3
+    assert abs(e18 - (-1.2214418066604806)) < 2e-5
+    assert abs(eps18 - (-0.4168423341628571)) < 1e-5
