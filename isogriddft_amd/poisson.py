@@ -524,6 +524,8 @@ def _diag_inverse(dg, pdata):
     """Inverse diagonal of -L on interior nodes (zero on faces)."""
     if pdata.dinv is not None:
         return pdata.dinv
+    if getattr(pdata, "dinv_monitor", None) is not None:
+        return pdata.dinv_monitor
     # monitor: the diagonal of -L is read off by applying the operator to unit impulses
     # at stride 5 in every axis (the 19-point stencil does not reach 5 nodes away)
     dev = dg.device
