@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--eig-tol", type=float, default=None,
                     help="eigensolver residual tolerance (default: 1e-3 for c2, 3e-4 for c5)")
     ap.add_argument("--eig-block", type=int, default=None,
-                    help="working-subspace size (default: the reference's 12k + 8 for c2, 4k + 8 for c5)")
+                    help="working-subspace size (default: the reference's 12k + 8 for c2, 4k for c5)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (the scale for PMC byte counters)")
     return ap.parse_args()
@@ -140,6 +140,8 @@ def solver_leg(geom, dg, cfg, veff64, eig_tol, order=2, block=None):
         out[f"eig_{prec}_converged"] = bool(res.converged)
         ev[prec] = np.asarray(res.eigenvalues)
         out[f"eig_{prec}_phase_s"] = {key: round(v, 4) for key, v in res.timings.items()}
+        out[f"eig_{prec}_reprojected_dropped_truncated"] = [res.reprojected_rows, res.dropped_rows,
+                                                          res.truncated_steps]
         print(f"[bench] eigensolve {prec}: {out[f'eig_{prec}_s']} s, {res.iterations} iterations",
               file=sys.stderr, flush=True)
     out["eig_fp32_iterations"] = res.fp32_iterations
@@ -258,7 +260,7 @@ def main():
     if not args.no_solvers:
         c2 = args.workload == "c2"
         tol = args.eig_tol or (1e-3 if c2 else 3e-4)
-        block = args.eig_block or (None if c2 else 4 * k + 8)
+        block = args.eig_block or (None if c2 else 4 * k)
         line["eigensolve_poisson"] = solver_leg(geom, dg, cfg, veff64, tol, order=2 if c2 else 4, block=block)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(geom, psi_np, veff64.cpu().numpy().reshape(dg.shape), args.cpu_seconds)

@@ -61,6 +61,27 @@ FP32_STALL_CEILING = 100.0
 # ------------------------------------------------------------------ weighted block algebra
 
 
+# widest column block handed to one FP32 library GEMM
+MM_PANEL = 1 << 20
+
+
+def _mm(A, B, out=None):
+    """A @ B for a small A (M, K) and a row-major block B (K, N).  FP32 products wider than
+    MM_PANEL columns run in column panels: the library's FP32 GEMM (hipBLASLt through
+    torch) was measured to return wrong values (NaN or ~1e17) for several (M, K), e.g.
+    (192, 248) and (104, 152), at N = 224^3 in one call, and correct ones per panel of
+    2^20 columns (FP64 and the rocBLAS path were correct at every shape tried)."""
+    N = B.shape[1]
+    if B.dtype != torch.float32 or N <= MM_PANEL:
+        return torch.matmul(A, B, out=out) if out is not None else torch.matmul(A, B)
+    if out is None:
+        out = torch.empty((A.shape[0], N), dtype=B.dtype, device=B.device)
+    for c0 in range(0, N, MM_PANEL):
+        c1 = min(c0 + MM_PANEL, N)
+        out[:, c0:c1].copy_(torch.matmul(A, B[:, c0:c1]))
+    return out
+
+
 def _sym(P):
     return 0.5 * (P + P.T)
 
@@ -76,7 +97,7 @@ def _whiten(C, w, floor=1e-10):
         L = torch.linalg.cholesky(G + 1e-14 * float(ev.abs().max()) * torch.eye(G.shape[0], dtype=G.dtype,
                                                                                   device=G.device))
         T = torch.linalg.inv(L).T  # C^T W C = L L^T  ->  (L^-1 C) is W-orthonormal
-    out = torch.matmul(T.T.to(C.dtype), C.reshape(C.shape[0], -1))
+    out = _mm(T.T.to(C.dtype), C.reshape(C.shape[0], -1))
     return out, T
 
 
@@ -86,7 +107,7 @@ def weighted_qr(X, w):
     G = _sym(gram(X, X, w))
     L = torch.linalg.cholesky(G)
     Linv = torch.linalg.inv(L)
-    return torch.matmul(Linv.to(X.dtype), X.reshape(X.shape[0], -1))
+    return _mm(Linv.to(X.dtype), X.reshape(X.shape[0], -1))
 
 
 def orthonormalize_device(X, w, rank_tolerance=1e-12, require_full_rank=True):
@@ -109,7 +130,7 @@ def orthonormalize_device(X, w, rank_tolerance=1e-12, require_full_rank=True):
         raise ValueError("The orbital block is rank-deficient under weighted orthonormalization: "
                          f"rank={rank}, requested={m}.")
     T = V[:, :rank] / s[None, :rank]
-    out = torch.matmul(T.T.to(X.dtype), X.reshape(m, -1))
+    out = _mm(T.T.to(X.dtype), X.reshape(m, -1))
     return out.reshape((rank,) + tuple(X.shape[1:]))
 
 
@@ -127,6 +148,11 @@ class BlockIterationResult:
     basis: torch.Tensor | None = None
     fp32_iterations: int = 0
     timings: dict = field(default_factory=dict)
+    # residual rows projected twice / dropped as dependent, and Rayleigh-Ritz steps solved
+    # on the range of a singular metric
+    reprojected_rows: int = 0
+    dropped_rows: int = 0
+    truncated_steps: int = 0
 
 
 def _host_geigh(A, B):
@@ -375,7 +401,7 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
     def rotate(Qm, src, dst, spare):
         """dst = Qm src, out of place into the spare buffer's rows or in column panels."""
         if spare is not None:
-            torch.matmul(Qm, src, out=dst)
+            _mm(Qm, src, out=dst)
             return
         for c0 in range(0, N, ROTATION_PANEL):
             c1 = min(c0 + ROTATION_PANEL, N)
@@ -419,8 +445,8 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
             theta, V = _host_geigh(P, Gzz)
             mark("eigh")
             Ct = V[:, :k].T.to(dev, dt).contiguous()
-            Psi = torch.matmul(Ct, Z)
-            torch.matmul(Ct, HZ, out=R)
+            Psi = _mm(Ct, Z)
+            _mm(Ct, HZ, out=R)
             th = theta[:k].numpy()
             R.sub_(torch.from_numpy(th).to(dev, dt)[:, None] * Psi)
             refresh = False
@@ -485,7 +511,7 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
         # project the residuals out of the basis and whiten them
         GRZ = gemm_nt(R, Z)
         mark("gram")
-        R.sub_(torch.matmul(GRZ.to(dt), Z))
+        R.sub_(_mm(GRZ.to(dt), Z))
         GR = gemm_nt(R, R).cpu()
         mark("gram")
         # |R_pre|^2 = |R|^2 + |R Z^T|^2 (Z orthonormal).  Rows that lost more than 99 % of
@@ -497,9 +523,10 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
         d_pre = d + (GRZ.cpu() ** 2).sum(1)
         again = d < 1e-4 * d_pre
         if bool(again.any()):
+            res.reprojected_rows += int(again.sum())
             idx = again.nonzero().flatten().to(dev)
             Rf = R.index_select(0, idx)
-            Rf.sub_(torch.matmul(gemm_nt(Rf, Z).to(dt), Z))
+            Rf.sub_(_mm(gemm_nt(Rf, Z).to(dt), Z))
             R.index_copy_(0, idx, Rf)
             del Rf
             GR = gemm_nt(R, R).cpu()
@@ -507,13 +534,14 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
             mark("gram")
         dead = d <= (DEPENDENT_REL * torch.finfo(dt).eps) ** 2 * d_pre
         if bool(dead.any()):
+            res.dropped_rows += int(dead.sum())
             R[dead.to(dev)] = 0
             GR[dead, :] = 0
             GR[:, dead] = 0
             GR[dead, dead] = 1.0
         Linv_r = _chol_inv(GR).to(dev)
         mark("eigh")
-        R.copy_(torch.matmul(Linv_r.to(dt), R))
+        R.copy_(_mm(Linv_r.to(dt), R))
         apply_s(R, HR)
         G23 = gemm_nt(BUF[:m + k], BUF[m:m + 2 * k]).cpu()  # [Z; R] [R; HR]^T: (m + k, 2k)
         # R HZ^T: (k, m); equal to (Z HR^T)^T up to rounding when S is symmetric
@@ -527,6 +555,8 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
         GB = _sym(torch.cat([torch.cat([Bz, ZR], 1), torch.cat([ZR.T, RR], 1)], 0))
         # generalized Rayleigh-Ritz on [Z; R]: c^T GB c = I, c^T T c = diag(theta)
         thT, c = _host_geigh(T, GB)
+        if c.shape[1] < m + k:
+            res.truncated_steps += 1
         if c.shape[1] < m:
             raise RuntimeError("the Rayleigh-Ritz space lost rank below the working-subspace size")
         theta_all = thT[:m].numpy().copy()
@@ -852,7 +882,7 @@ def _cell_volume_orthonormal(H, dg, X, eigenvalues):
     w = dg.weights()
     G = 0.5 * (gram(X, X, w) + gram(X, X, w).T)
     Linv = torch.linalg.inv(torch.linalg.cholesky(G))
-    Xo = torch.matmul(Linv.to(X.dtype), X.reshape(X.shape[0], -1)).reshape(X.shape)
+    Xo = _mm(Linv.to(X.dtype), X.reshape(X.shape[0], -1)).reshape(X.shape)
     HX = torch.empty_like(Xo)
     H.apply(Xo.view((-1,) + dg.shape), HX.view((-1,) + dg.shape))
     lam = torch.as_tensor(np.asarray(eigenvalues, dtype=np.float64), device=X.device)

@@ -136,3 +136,27 @@ def test_row_wise_cg_kernels(dt, N):
     cg_direction(p, r, dinv, beta)
     pw = dinv * r + beta.to(dt)[:, None] * p0
     assert torch.allclose(p, pw, rtol=0, atol=4 * torch.finfo(dt).eps * pw.abs().max().item())
+
+
+def test_fp32_block_products_at_large_n():
+    """(M, K) x (K, N) FP32 products at N = 224^3 through the eigensolver's panelled GEMM:
+    (192, 248) is one of the shapes for which a single library FP32 GEMM call was measured
+    to return wrong values there; the panelled product must match FP64 to FP32 rounding."""
+    from isogriddft_amd.eigensolver import _mm
+
+    N = 224 ** 3
+    g = torch.Generator(device="cuda")
+    g.manual_seed(11)
+    for M, K in ((192, 248), (48, 200)):
+        B = torch.randn((K, N), dtype=torch.float32, device="cuda", generator=g)
+        A = torch.randn((M, K), dtype=torch.float32, device="cuda", generator=g)
+        got = _mm(A, B)
+        ref = A.double() @ B.double()
+        err = float((got.double() - ref).abs().max() / ref.abs().max())
+        print(M, K, err)
+        assert err < 1e-5
+        out = torch.empty((M + 8, N), dtype=torch.float32, device="cuda")
+        _mm(A, B, out=out[:M])
+        assert torch.equal(out[:M], got)
+        del B, A, got, ref, out
+        torch.cuda.empty_cache()
