@@ -160,3 +160,21 @@ def test_fp32_block_products_at_large_n():
         assert torch.equal(out[:M], got)
         del B, A, got, ref, out
         torch.cuda.empty_cache()
+
+
+def test_fp32_split_k_products_at_large_n():
+    """The eigensolver's split-K Gram products in FP32 at N = 224^3 (one strided-batched
+    library GEMM over 2^8 K panels) against FP64, for the shapes of a k = 48, m = 192
+    iteration."""
+    from isogriddft_amd.eigensolver import _gemm_nt_split
+
+    N = 224 ** 3
+    g = torch.Generator(device="cuda")
+    g.manual_seed(12)
+    buf = torch.randn((288, N), dtype=torch.float32, device="cuda", generator=g)
+    for (a0, a1, b0, b1) in ((0, 240, 192, 288), (192, 240, 0, 192), (192, 240, 192, 240)):
+        got = _gemm_nt_split(buf[a0:a1], buf[b0:b1])
+        ref = buf[a0:a1].double() @ buf[b0:b1].double().T
+        err = float((got - ref).abs().max() / ref.abs().max())
+        print((a0, a1, b0, b1), err)
+        assert err < 1e-5
