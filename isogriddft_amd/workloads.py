@@ -102,7 +102,7 @@ def run(name="benzene", n=224, k=None, block=None, tol=3e-4, multipole_order=4, 
     plan = pseudo.NonlocalPlan(dg, atoms)
     out["projector_setup_s"] = round(time.perf_counter() - t0, 3)
     H = BlockHamiltonian(dg, veff, closure="trial_fix", nonlocal_plan=plan)
-    m = block or 4 * k + 8
+    m = block or 4 * k
     X0 = eigensolver.default_guess_block(geom, atoms, m, dg)
     _log(f"starting block of {m} orbitals built")
     ev = {}
@@ -189,7 +189,7 @@ def main(argv=None):
     ap.add_argument("molecule", choices=sorted(MOLECULES) + ["h2_legacy"])
     ap.add_argument("--n", type=int, default=224, help="grid points per axis")
     ap.add_argument("--k", type=int, default=None, help="states (default: the molecule's k)")
-    ap.add_argument("--block", type=int, default=None, help="working subspace (default 4k + 8)")
+    ap.add_argument("--block", type=int, default=None, help="working subspace (default 4k)")
     ap.add_argument("--tol", type=float, default=3e-4, help="eigensolver residual tolerance")
     args = ap.parse_args(argv)
     if not torch.cuda.is_available():
