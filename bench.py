@@ -11,6 +11,11 @@ Prints one JSON line:
   roofline       achieved vs 8 TB/s for the stencil kernel, timed with HIP events on
                  the launch stream; `traffic` from the committed rocprofv3 PMC summary
   cpu_baseline   the FP64 NumPy oracle applying the same block on this host
+  eigensolve_poisson          the Poisson solve and lowest-16 eigensolve on the same grid,
+                              FP64 against FP32 + FP64 refinement
+  eigensolve_poisson_benzene  the metric's second part on its own configuration: benzene
+                              on the 224^3 monitor grid, k = 48, eigensolve + Poisson in
+                              FP64 and in FP32 + FP64 refinement, and the adjoint forces
 Algorithmic bytes per block apply: B = N (2 s_psi k + 7 s_g + s_V).
 """
 
@@ -53,6 +58,9 @@ def parse():
                     help="eigensolver residual tolerance (default: 1e-3 for c2, 3e-4 for c5)")
     ap.add_argument("--eig-block", type=int, default=None,
                     help="working-subspace size (default: the reference's 12k + 8 for c2, 4k for c5)")
+    ap.add_argument("--molecule", choices=("benzene", "h2o", "none"), default="benzene",
+                    help="molecular leg: the fixed-potential eigensolve + Poisson of the molecule on its "
+                         "monitor grid, FP64 against FP32 + FP64 refinement (benzene: 224^3, k=48)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (the scale for PMC byte counters)")
     return ap.parse_args()
@@ -151,6 +159,31 @@ def solver_leg(geom, dg, cfg, veff64, eig_tol, order=2, block=None):
     out["workload"] = (f"{dg.shape[0]}^3 grid, Poisson: {cfg.n_charges} Gaussian charges, order-{order} multipoles, "
                        f"rtol 1e-8; eigensolve: k={k}, working subspace {m}, residual tol {eig_tol}, "
                        "trial_fix closure, V = two Z=3 soft-Coulomb centres + v_H")
+    return out
+
+
+MOLECULE_LEGS = {"benzene": dict(n=224, k=48, block=192, tol=3e-4), "h2o": dict(n=128, k=8, block=None, tol=3e-4)}
+
+
+def molecule_leg(name):
+    """The metric's second part on its own configuration: the molecule's fixed-potential
+    eigensolve plus open-boundary Poisson solve (multipole order 4) on its atom-centred
+    monitor grid, FP64 against FP32 + FP64 refinement, then the adjoint forces at the
+    FP64 orbitals (``isogriddft_amd.workloads.run``)."""
+    from isogriddft_amd import workloads
+
+    kw = MOLECULE_LEGS[name]
+    r = workloads.run(name, kw["n"], kw["k"], kw["block"], kw["tol"])
+    keys = ("workload", "k", "occupied", "grid_s", "poisson_fp64_s", "poisson_mixed_s", "poisson_fp64_iterations",
+            "poisson_mixed_iterations", "poisson_mixed_vs_fp64_max_rel", "eig_fp64_s", "eig_mixed_s",
+            "eig_fp64_iterations", "eig_mixed_iterations", "eig_fp32_iterations", "eig_fp64_converged",
+            "eig_mixed_converged", "eig_fp64_phase_s", "eig_mixed_phase_s", "eig_mixed_vs_fp64_max_abs_Ha",
+            "eigensolve_plus_poisson_fp64_s", "eigensolve_plus_poisson_mixed_s", "forces_s",
+            "forces_net_Ha_per_Bohr")
+    out = {key: r[key] for key in keys if key in r}
+    out["working_subspace"] = kw["block"] or 4 * kw["k"]
+    out["residual_tol"] = kw["tol"]
+    out["mixed_speedup"] = round(r["eigensolve_plus_poisson_fp64_s"] / r["eigensolve_plus_poisson_mixed_s"], 3)
     return out
 
 
@@ -262,6 +295,9 @@ def main():
         tol = args.eig_tol or (1e-3 if c2 else 3e-4)
         block = args.eig_block or (None if c2 else 4 * k)
         line["eigensolve_poisson"] = solver_leg(geom, dg, cfg, veff64, tol, order=2 if c2 else 4, block=block)
+        if args.molecule != "none":
+            torch.cuda.empty_cache()
+            line[f"eigensolve_poisson_{args.molecule}"] = molecule_leg(args.molecule)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(geom, psi_np, veff64.cpu().numpy().reshape(dg.shape), args.cpu_seconds)
     print(json.dumps(line), flush=True)
