@@ -307,7 +307,7 @@ int ig_mapping_metric_f64(const double* coords, int nx, int ny, int nz, double h
 
 /* ------------------------------------------------------------------ row-wise block CG */
 
-/* Doubles of `work` needed by the three row-wise entries below for k rows. */
+/* Doubles of `work` needed by the row-wise entries below that return sums, for k rows. */
 int ig_row_dots_work_size(int k);
 /* out[i] = sum_n w[n] a[i][n] b[i][n] (w NULL: 1) for i < k, rows of length N at stride ld;
  * FP64 accumulation, deterministic.  No reference counterpart: the reference's block
@@ -330,6 +330,21 @@ int ig_cg_direction_f64(double* p, const double* r, const double* dinv, const do
                         long long ld, hipStream_t stream);
 int ig_cg_direction_f32(float* p, const float* r, const float* dinv, const double* beta, int k, long long N,
                         long long ld, hipStream_t stream);
+/* Start of the row-wise CG from x0 = dinv isw b with q = A x0 already applied: row i:
+ * r = isw b - q, p = dinv r, out[i] = sum_n w[n] r[i][n] p[i][n].  b has row stride ldb,
+ * the other blocks ld; isw, dinv, w hold N values.  No reference counterpart (see above). */
+int ig_cg_start_f64(const double* b, long long ldb, const double* isw, const double* q, const double* dinv,
+                    const double* w, double* r, double* p, int k, long long N, long long ld, double* work,
+                    double* out, hipStream_t stream);
+int ig_cg_start_f32(const float* b, long long ldb, const float* isw, const float* q, const float* dinv,
+                    const float* w, float* r, float* p, int k, long long N, long long ld, double* work, double* out,
+                    hipStream_t stream);
+/* Last CG step mapped back: row i: y = sw (x + alpha[i] p), y at row stride ldy (x is not
+ * written).  No reference counterpart (see above). */
+int ig_cg_finish_f64(const double* x, const double* p, const double* sw, const double* alpha, double* y,
+                     long long ldy, int k, long long N, long long ld, hipStream_t stream);
+int ig_cg_finish_f32(const float* x, const float* p, const float* sw, const double* alpha, float* y, long long ldy,
+                     int k, long long N, long long ld, hipStream_t stream);
 
 /* ------------------------------------------------------------------ Green identity */
 

@@ -76,6 +76,10 @@ SIGNATURES = {
     "ig_cg_update_f32": [P, P, P, P, P, P, P, I, L, L, P, P, P],
     "ig_cg_direction_f64": [P, P, P, P, I, L, L, P],
     "ig_cg_direction_f32": [P, P, P, P, I, L, L, P],
+    "ig_cg_start_f64": [P, L, P, P, P, P, P, P, I, L, L, P, P, P],
+    "ig_cg_start_f32": [P, L, P, P, P, P, P, P, I, L, L, P, P, P],
+    "ig_cg_finish_f64": [P, P, P, P, P, L, I, L, L, P],
+    "ig_cg_finish_f32": [P, P, P, P, P, L, I, L, L, P],
     "ig_green_identity_work_size": [],
     "ig_green_identity_f64": [P, P, P, P, P, I, I, I, D, D, D, I, P, P, P],
 }

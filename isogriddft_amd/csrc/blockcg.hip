@@ -6,6 +6,9 @@
 //   row_dots:     out[i] = sum_n w[n] a[i][n] b[i][n]
 //   cg_update:    x += alpha_i p, r -= alpha_i q, out[i] = sum_n w[n] dinv[n] r[i][n]^2
 //   cg_direction: p = dinv r + beta_i p
+//   cg_start:     r = isw b - q, p = dinv r, out[i] = sum_n w[n] r[i][n] p[i][n]
+//                 (the first residual of x0 = dinv isw b, with q = A x0)
+//   cg_finish:    y = sw (x + alpha_i p) (the last x update mapped back, x not written)
 // alpha / beta are FP64 device vectors of length k (no host round trip).  Sums are FP64;
 // each (row, chunk) block writes a partial and one block per row adds the partials in a
 // fixed order, so results are bit-reproducible.
@@ -198,6 +201,118 @@ __global__ __launch_bounds__(IG_THREADS) void cg_direction_kernel(T* __restrict_
   for (long long n = beg + threadIdx.x; n < end; n += IG_THREADS) pr[n] = dinv[n] * rr[n] + b * pr[n];
 }
 
+template <typename T>
+__global__ __launch_bounds__(IG_THREADS) void cg_start_vec(const T* __restrict__ b, long long ldb,
+                                                           const T* __restrict__ isw, const T* __restrict__ q,
+                                                           const T* __restrict__ dinv, const T* __restrict__ w,
+                                                           T* __restrict__ r, T* __restrict__ p, long long N,
+                                                           long long ld, long long chunk,
+                                                           double* __restrict__ partial) {
+  using VT = typename Vec16<T>::t;
+  constexpr int VW = Vec16<T>::W;
+  __shared__ double sm[4];
+  const int row = blockIdx.y;
+  const long long beg = (long long)blockIdx.x * chunk;
+  const long long end = min(beg + chunk, N);
+  const T* br = b + row * ldb;
+  const T* qr = q + row * ld;
+  T* rr = r + row * ld;
+  T* pr = p + row * ld;
+  double s = 0.0;
+  for (long long n = beg + (long long)threadIdx.x * VW; n < end; n += (long long)IG_THREADS * VW) {
+    const VT vb = *reinterpret_cast<const VT*>(br + n);
+    const VT vs = *reinterpret_cast<const VT*>(isw + n);
+    const VT vq = *reinterpret_cast<const VT*>(qr + n);
+    const VT vd = *reinterpret_cast<const VT*>(dinv + n);
+    VT vw;
+    if (w) vw = *reinterpret_cast<const VT*>(w + n);
+    VT vr, vp;
+#pragma unroll
+    for (int e = 0; e < VW; ++e) {
+      vr[e] = vs[e] * vb[e] - vq[e];
+      vp[e] = vd[e] * vr[e];
+      const double t = (double)vr[e] * (double)vp[e];
+      s += w ? t * (double)vw[e] : t;
+    }
+    *reinterpret_cast<VT*>(rr + n) = vr;
+    *reinterpret_cast<VT*>(pr + n) = vp;
+  }
+  double v[1] = {s};
+  ig_block_sum<1>(v, sm);
+  if (threadIdx.x == 0) partial[(long long)row * gridDim.x + blockIdx.x] = v[0];
+}
+
+template <typename T>
+__global__ __launch_bounds__(IG_THREADS) void cg_start_kernel(const T* __restrict__ b, long long ldb,
+                                                              const T* __restrict__ isw, const T* __restrict__ q,
+                                                              const T* __restrict__ dinv, const T* __restrict__ w,
+                                                              T* __restrict__ r, T* __restrict__ p, long long N,
+                                                              long long ld, long long chunk,
+                                                              double* __restrict__ partial) {
+  __shared__ double sm[4];
+  const int row = blockIdx.y;
+  const long long beg = (long long)blockIdx.x * chunk;
+  const long long end = min(beg + chunk, N);
+  const T* br = b + row * ldb;
+  const T* qr = q + row * ld;
+  T* rr = r + row * ld;
+  T* pr = p + row * ld;
+  double s = 0.0;
+  for (long long n = beg + threadIdx.x; n < end; n += IG_THREADS) {
+    const T rn = isw[n] * br[n] - qr[n];
+    const T pn = dinv[n] * rn;
+    rr[n] = rn;
+    pr[n] = pn;
+    const double t = (double)rn * (double)pn;
+    s += w ? t * (double)w[n] : t;
+  }
+  double v[1] = {s};
+  ig_block_sum<1>(v, sm);
+  if (threadIdx.x == 0) partial[(long long)row * gridDim.x + blockIdx.x] = v[0];
+}
+
+template <typename T>
+__global__ __launch_bounds__(IG_THREADS) void cg_finish_vec(const T* __restrict__ x, const T* __restrict__ p,
+                                                            const T* __restrict__ sw,
+                                                            const double* __restrict__ alpha, T* __restrict__ y,
+                                                            long long ldy, long long N, long long ld,
+                                                            long long chunk) {
+  using VT = typename Vec16<T>::t;
+  constexpr int VW = Vec16<T>::W;
+  const int row = blockIdx.y;
+  const long long beg = (long long)blockIdx.x * chunk;
+  const long long end = min(beg + chunk, N);
+  const T a = (T)alpha[row];
+  const T* xr = x + row * ld;
+  const T* pr = p + row * ld;
+  T* yr = y + row * ldy;
+  for (long long n = beg + (long long)threadIdx.x * VW; n < end; n += (long long)IG_THREADS * VW) {
+    const VT vx = *reinterpret_cast<const VT*>(xr + n);
+    const VT vp = *reinterpret_cast<const VT*>(pr + n);
+    const VT vs = *reinterpret_cast<const VT*>(sw + n);
+    VT vy;
+#pragma unroll
+    for (int e = 0; e < VW; ++e) vy[e] = vs[e] * (vx[e] + a * vp[e]);
+    *reinterpret_cast<VT*>(yr + n) = vy;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(IG_THREADS) void cg_finish_kernel(const T* __restrict__ x, const T* __restrict__ p,
+                                                               const T* __restrict__ sw,
+                                                               const double* __restrict__ alpha, T* __restrict__ y,
+                                                               long long ldy, long long N, long long ld,
+                                                               long long chunk) {
+  const int row = blockIdx.y;
+  const long long beg = (long long)blockIdx.x * chunk;
+  const long long end = min(beg + chunk, N);
+  const T a = (T)alpha[row];
+  const T* xr = x + row * ld;
+  const T* pr = p + row * ld;
+  T* yr = y + row * ldy;
+  for (long long n = beg + threadIdx.x; n < end; n += IG_THREADS) yr[n] = sw[n] * (xr[n] + a * pr[n]);
+}
+
 __global__ void rows_finish(const double* __restrict__ partial, int nparts, int k, double* __restrict__ out) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += gridDim.x * blockDim.x) {
     double s = 0.0;
@@ -211,8 +326,22 @@ struct RowPlan {
   long long chunk;
 };
 
+// Workgroups of these kernels resident on the device at once: every one runs at 8 waves
+// per SIMD, i.e. 8 workgroups of 256 lanes per CU.
+long long resident_groups() {
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+    cus = 256;
+  return 8LL * cus;
+}
+
+// k rows split into chunks so that the k x nch workgroups fit in one residency round:
+// all of them start together and carry equal work, so none is left to stream its chunk
+// alone after the rest have finished (ceil(2048 / k) chunks overshoot one round for most
+// k, e.g. 2064 workgroups for k = 48 against 2048 resident).
 RowPlan row_plan(int k, long long N, int align) {
-  long long nch = (2048 + k - 1) / k;
+  long long nch = resident_groups() / k;
   const long long cap = (N + IG_THREADS - 1) / IG_THREADS;  // at least one element per thread
   if (nch > cap) nch = cap;
   if (nch > CG_MAX_CHUNKS) nch = CG_MAX_CHUNKS;
@@ -284,6 +413,41 @@ int cg_direction(T* p, const T* r, const T* dinv, const double* beta, int k, lon
   return IG_OK;
 }
 
+template <typename T>
+int cg_start(const T* b, long long ldb, const T* isw, const T* q, const T* dinv, const T* w, T* r, T* p, int k,
+             long long N, long long ld, double* work, double* out, hipStream_t s) {
+  if (!b || !isw || !q || !dinv || !r || !p || !work || !out || k <= 0 || k > 65535 || N <= 0 || ld < N ||
+      ldb < N)
+    return IG_EINVAL;
+  const bool vec = use_vec<T>(N, ld, {b, isw, q, dinv, w, r, p}) && ldb % Vec16<T>::W == 0;
+  const RowPlan g = row_plan(k, N, vec ? Vec16<T>::W : 1);
+  if (vec)
+    hipLaunchKernelGGL(cg_start_vec<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, b, ldb, isw, q, dinv, w, r, p, N,
+                       ld, g.chunk, work);
+  else
+    hipLaunchKernelGGL(cg_start_kernel<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, b, ldb, isw, q, dinv, w, r, p,
+                       N, ld, g.chunk, work);
+  hipLaunchKernelGGL(rows_finish, dim3((k + 255) / 256), dim3(256), 0, s, work, g.nch, k, out);
+  IG_CHECK_LAUNCH();
+  return IG_OK;
+}
+
+template <typename T>
+int cg_finish(const T* x, const T* p, const T* sw, const double* alpha, T* y, long long ldy, int k, long long N,
+              long long ld, hipStream_t s) {
+  if (!x || !p || !sw || !alpha || !y || k <= 0 || k > 65535 || N <= 0 || ld < N || ldy < N) return IG_EINVAL;
+  const bool vec = use_vec<T>(N, ld, {x, p, sw, y}) && ldy % Vec16<T>::W == 0;
+  const RowPlan g = row_plan(k, N, vec ? Vec16<T>::W : 1);
+  if (vec)
+    hipLaunchKernelGGL(cg_finish_vec<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, x, p, sw, alpha, y, ldy, N, ld,
+                       g.chunk);
+  else
+    hipLaunchKernelGGL(cg_finish_kernel<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, x, p, sw, alpha, y, ldy, N, ld,
+                       g.chunk);
+  IG_CHECK_LAUNCH();
+  return IG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -315,6 +479,25 @@ int ig_cg_direction_f64(double* p, const double* r, const double* dinv, const do
 int ig_cg_direction_f32(float* p, const float* r, const float* dinv, const double* beta, int k, long long N,
                         long long ld, hipStream_t s) {
   return cg_direction<float>(p, r, dinv, beta, k, N, ld, s);
+}
+
+int ig_cg_start_f64(const double* b, long long ldb, const double* isw, const double* q, const double* dinv,
+                    const double* w, double* r, double* p, int k, long long N, long long ld, double* work,
+                    double* out, hipStream_t s) {
+  return cg_start<double>(b, ldb, isw, q, dinv, w, r, p, k, N, ld, work, out, s);
+}
+int ig_cg_start_f32(const float* b, long long ldb, const float* isw, const float* q, const float* dinv,
+                    const float* w, float* r, float* p, int k, long long N, long long ld, double* work, double* out,
+                    hipStream_t s) {
+  return cg_start<float>(b, ldb, isw, q, dinv, w, r, p, k, N, ld, work, out, s);
+}
+int ig_cg_finish_f64(const double* x, const double* p, const double* sw, const double* alpha, double* y,
+                     long long ldy, int k, long long N, long long ld, hipStream_t s) {
+  return cg_finish<double>(x, p, sw, alpha, y, ldy, k, N, ld, s);
+}
+int ig_cg_finish_f32(const float* x, const float* p, const float* sw, const double* alpha, float* y, long long ldy,
+                     int k, long long N, long long ld, hipStream_t s) {
+  return cg_finish<float>(x, p, sw, alpha, y, ldy, k, N, ld, s);
 }
 
 }  // extern "C"

@@ -51,7 +51,7 @@ from .device import DeviceGrid, device_grid
 from .hamiltonian import (BlockHamiltonian, H2_BENCHMARK_CASE, normalize_kinetic_version, normalize_spin_channel,
                           resolve_hartree, resolve_xc)
 from .kinetic import apply_operator
-from .reductions import cg_direction, cg_update, gram, row_dots
+from .reductions import cg_direction, cg_finish, cg_start, cg_update, gram, row_dots
 
 FP32_RESIDUAL_FLOOR = 5e-5
 # an FP32 plateau promotes early only within this factor of the FP32 floor
@@ -193,8 +193,8 @@ def _chol_inv(G):
 
 PRECONDITIONER_FLOOR = 1e-2
 # kinetic preconditioner: CG steps per residual block and the SPD shift c (Ha)
-KINETIC_CG_STEPS = 6
-KINETIC_SHIFT = 0.5
+KINETIC_CG_STEPS = 8
+KINETIC_SHIFT = 1.0
 # residual rows whose (twice) projection out of the basis keeps less than DEPENDENT_REL *
 # eps of their norm are treated as lying in the basis span
 DEPENDENT_REL = 10.0
@@ -346,25 +346,29 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
         dtb = Rb.dtype
         if dtb not in cg_bufs:
             cg_bufs.clear()
-            cg_bufs[dtb] = (torch.empty((4, k, N), dtype=dtb, device=dev), kdinv64.to(dtb), w64.to(dtb))
-        buf, dinv, wd = cg_bufs[dtb]
+            dinv_t = kdinv64.to(dtb)
+            isw_t = (1.0 / sw64).to(dtb)
+            cg_bufs[dtb] = (torch.empty((4, k, N), dtype=dtb, device=dev), dinv_t, w64.to(dtb), isw_t,
+                            isw_t * dinv_t, sw64.to(dtb))
+        buf, dinv, wd, iswd, isd, swd = cg_bufs[dtb]
         x, r, pv, q = buf[0], buf[1], buf[2], buf[3]
         mark("rotation")
-        torch.mul(Rb, isw, out=r)                       # r = W^(-1/2) R
-        torch.mul(r, dinv, out=x)                       # x0 = D^-1 r
+        torch.mul(Rb, isd, out=x)                       # x0 = D^-1 W^(-1/2) R
         kop.apply(x.view((k,) + shape), q.view((k,) + shape))
-        r.sub_(q)
-        torch.mul(r, dinv, out=pv)
-        rz = row_dots(r, pv, wd)
-        for _ in range(KINETIC_CG_STEPS):
+        rz = cg_start(Rb, iswd, q, dinv, wd, r, pv)     # r = W^(-1/2) R - A x0, p = D^-1 r
+        for step in range(KINETIC_CG_STEPS):
             kop.apply(pv.view((k,) + shape), q.view((k,) + shape))
             pq = row_dots(pv, q, wd)
             alpha = torch.where(pq > 0, rz / torch.where(pq > 0, pq, 1.0), 0.0)
+            if step == KINETIC_CG_STEPS - 1:
+                cg_finish(x, pv, swd, alpha, Rb)        # R = W^(1/2) (x + alpha p)
+                break
             rz_new = cg_update(x, r, pv, q, dinv, wd, alpha)
             beta = torch.where(rz > 0, rz_new / torch.where(rz > 0, rz, 1.0), 0.0)
             cg_direction(pv, r, dinv, beta)
             rz = rz_new
-        torch.mul(x, sw, out=Rb)
+        else:
+            torch.mul(x, swd, out=Rb)                   # no CG step: R = W^(1/2) x0
         mark("precondition")
 
     def gemm_nt(A, B):

@@ -98,12 +98,30 @@ def _rows(t):
     return t
 
 
+def _field(f, like, name, optional=False):
+    """A per-point vector (N values, contiguous, the blocks' dtype) shared by every row."""
+    if f is None:
+        if optional:
+            return None
+        raise ValueError(f"{name} is required")
+    if f.dim() != 1 or f.numel() != like.shape[1] or not f.is_contiguous() or f.dtype != like.dtype:
+        raise ValueError(f"{name} must be a contiguous vector of N = {like.shape[1]} values of dtype {like.dtype}")
+    return f
+
+
+def _alpha(a, k, name):
+    if a.dtype != torch.float64 or a.numel() != k or not a.is_contiguous():
+        raise ValueError(f"{name} must be a contiguous FP64 device vector of length k = {k}")
+    return a
+
+
 def row_dots(a: torch.Tensor, b: torch.Tensor, w: torch.Tensor | None = None, stream=None) -> torch.Tensor:
     """FP64 device vector out[i] = sum_n w[n] a[i, n] b[i, n] (w None: 1)."""
     a, b = _rows(a), _rows(b)
     k, N = a.shape
     if b.shape != a.shape or b.stride(0) != a.stride(0) or b.dtype != a.dtype:
         raise ValueError("row_dots operands must share shape, row stride and dtype")
+    _field(w, a, "w", optional=True)
     lib = _lib.load()
     work = workspace(a.device, lib.ig_row_dots_work_size(k), "rows")
     out = torch.empty(k, dtype=torch.float64, device=a.device)
@@ -120,6 +138,7 @@ def cg_update(x, r, p, q, dinv, w, alpha, stream=None) -> torch.Tensor:
         if t.shape != x.shape or t.stride(0) != x.stride(0) or t.dtype != x.dtype:
             raise ValueError("cg_update blocks must share shape, row stride and dtype")
     k, N = x.shape
+    _field(dinv, x, "dinv"), _field(w, x, "w", optional=True), _alpha(alpha, k, "alpha")
     lib = _lib.load()
     work = workspace(x.device, lib.ig_row_dots_work_size(k), "rows")
     out = torch.empty(k, dtype=torch.float64, device=x.device)
@@ -136,6 +155,43 @@ def cg_direction(p, r, dinv, beta, stream=None) -> None:
     k, N = p.shape
     if r.shape != p.shape or r.stride(0) != p.stride(0) or r.dtype != p.dtype:
         raise ValueError("cg_direction blocks must share shape, row stride and dtype")
+    _field(dinv, p, "dinv"), _alpha(beta, k, "beta")
     sfx = "f64" if p.dtype == torch.float64 else "f32"
     _lib.call(f"ig_cg_direction_{sfx}", _lib.ptr(p), _lib.ptr(r), _lib.ptr(dinv), _lib.ptr(beta), k, N,
               p.stride(0), _lib.stream_handle(stream))
+
+
+def cg_start(b, isw, q, dinv, w, r, p, stream=None) -> torch.Tensor:
+    """First residual of the row-wise CG from x0 = dinv isw b (q = A x0 given): row i:
+    r = isw b - q, p = dinv r; returns the FP64 vector sum w r_i p_i."""
+    for t in (q, r, p):
+        _rows(t)
+        if t.shape != q.shape or t.stride(0) != q.stride(0) or t.dtype != q.dtype:
+            raise ValueError("cg_start blocks q, r, p must share shape, row stride and dtype")
+    _rows(b)
+    if b.shape != q.shape or b.dtype != q.dtype:
+        raise ValueError("cg_start: b must match q in shape and dtype")
+    k, N = q.shape
+    _field(isw, q, "isw"), _field(dinv, q, "dinv"), _field(w, q, "w", optional=True)
+    lib = _lib.load()
+    work = workspace(q.device, lib.ig_row_dots_work_size(k), "rows")
+    out = torch.empty(k, dtype=torch.float64, device=q.device)
+    sfx = "f64" if q.dtype == torch.float64 else "f32"
+    _lib.call(f"ig_cg_start_{sfx}", _lib.ptr(b), b.stride(0), _lib.ptr(isw), _lib.ptr(q), _lib.ptr(dinv),
+              _lib.ptr(w), _lib.ptr(r), _lib.ptr(p), k, N, q.stride(0), _lib.ptr(work), _lib.ptr(out),
+              _lib.stream_handle(stream))
+    return out
+
+
+def cg_finish(x, p, sw, alpha, y, stream=None) -> None:
+    """Last CG update mapped back: row i: y = sw (x + alpha_i p); x is not written."""
+    _rows(x), _rows(p), _rows(y)
+    if p.shape != x.shape or p.stride(0) != x.stride(0) or p.dtype != x.dtype:
+        raise ValueError("cg_finish blocks x, p must share shape, row stride and dtype")
+    if y.shape != x.shape or y.dtype != x.dtype:
+        raise ValueError("cg_finish: y must match x in shape and dtype")
+    k, N = x.shape
+    _field(sw, x, "sw"), _alpha(alpha, k, "alpha")
+    sfx = "f64" if x.dtype == torch.float64 else "f32"
+    _lib.call(f"ig_cg_finish_{sfx}", _lib.ptr(x), _lib.ptr(p), _lib.ptr(sw), _lib.ptr(alpha), _lib.ptr(y),
+              y.stride(0), k, N, x.stride(0), _lib.stream_handle(stream))

@@ -58,6 +58,22 @@ def test_block_iteration_matches_dense_eigh(k, precondition):
     assert torch.max(torch.abs(S - torch.eye(k, device=S.device, dtype=S.dtype))).item() <= 1e-12
 
 
+@pytest.mark.parametrize("steps", [0, 1, 2])
+def test_kinetic_preconditioner_step_counts(steps, monkeypatch):
+    """The kinetic preconditioner with no CG step (x0 = D^-1 r only), one step (the fused
+    start and finish back to back) and two steps converges to the dense eigenvalues."""
+    monkeypatch.setattr(eigensolver, "KINETIC_CG_STEPS", steps)
+    g, V = _problem()
+    dg = DeviceGrid(g)
+    H = BlockHamiltonian(dg, torch.from_numpy(V).cuda(), closure="trial_fix")
+    k = 3
+    X0 = _start(dg, g, eigensolver.reference_block_size(k))
+    res = eigensolver.block_subspace_iteration(H, dg.weights(), X0, k, 1e-7, 400)
+    print("steps", steps, "iters", res.iterations)
+    assert res.converged
+    assert np.max(np.abs(res.eigenvalues - _dense_eigs(g, V, k))) <= 1e-8
+
+
 @pytest.mark.parametrize("precondition", ["diagonal", "kinetic"])
 def test_mixed_precision_refinement_matches_fp64(precondition):
     g, V = _problem((14, 15, 16))

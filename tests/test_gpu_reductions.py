@@ -138,6 +138,51 @@ def test_row_wise_cg_kernels(dt, N):
     assert torch.allclose(p, pw, rtol=0, atol=4 * torch.finfo(dt).eps * pw.abs().max().item())
 
 
+@pytest.mark.parametrize("dt", [torch.float64, torch.float32])
+@pytest.mark.parametrize("N", [70000, 70001])
+def test_cg_start_and_finish(dt, N):
+    """The fused first residual (r = isw b - q, p = dinv r, sum w r p) and the fused last
+    update (y = sw (x + alpha p)) against torch, on ragged blocks whose row strides differ
+    (b and y: stride 70008; q, r, p, x: 70016); N = 70001 takes the scalar kernels."""
+    from isogriddft_amd.reductions import cg_finish, cg_start
+
+    k = 7
+    g = torch.Generator(device="cuda")
+    g.manual_seed(11)
+
+    def block(ld):
+        return torch.randn((k, ld), dtype=torch.float64, device="cuda", generator=g).to(dt)[:, :N]
+
+    b, q, x, p = block(70008), block(70016), block(70016), block(70016)
+    r, pv = block(70016), block(70016)
+    w = torch.rand(N, dtype=torch.float64, device="cuda", generator=g).to(dt) + 0.5
+    dinv = torch.rand(N, dtype=torch.float64, device="cuda", generator=g).to(dt) + 0.1
+    isw = torch.rand(N, dtype=torch.float64, device="cuda", generator=g).to(dt) + 0.2
+    eps = torch.finfo(dt).eps
+    rz = cg_start(b, isw, q, dinv, w, r, pv)
+    rw = isw * b - q
+    assert torch.allclose(r, rw, rtol=0, atol=4 * eps * rw.abs().max().item())
+    pw = dinv * r
+    assert torch.allclose(pv, pw, rtol=0, atol=2 * eps * pw.abs().max().item())
+    rzw = (r.double() * pv.double() * w.double()).sum(1)
+    assert torch.allclose(rz, rzw, rtol=1e-13 if dt == torch.float64 else 1e-6, atol=0)
+    assert torch.equal(cg_start(b, isw, q, dinv, w, r, pv), rz)  # deterministic
+    rz0 = cg_start(b, isw, q, dinv, None, r, pv)
+    assert torch.allclose(rz0, (r.double() * pv.double()).sum(1), rtol=1e-13 if dt == torch.float64 else 1e-6)
+
+    alpha = torch.linspace(-0.6, 1.1, k, dtype=torch.float64, device="cuda")
+    y = block(70008)
+    x0 = x.clone()
+    cg_finish(x, p, isw, alpha, y)
+    yw = isw * (x0 + alpha.to(dt)[:, None] * p)
+    assert torch.equal(x, x0)
+    assert torch.allclose(y, yw, rtol=0, atol=4 * eps * yw.abs().max().item())
+    with pytest.raises(ValueError):
+        cg_finish(x, p, isw[:-1], alpha, y)
+    with pytest.raises(ValueError):
+        cg_start(b, isw, q, dinv.to(torch.float64 if dt == torch.float32 else torch.float32), w, r, pv)
+
+
 def test_fp32_block_products_at_large_n():
     """(M, K) x (K, N) FP32 products at N = 224^3 through the eigensolver's panelled GEMM:
     (192, 248) is one of the shapes for which a single library FP32 GEMM call was measured
