@@ -482,9 +482,15 @@ template <typename T>
 __device__ __forceinline__ typename ZVec<T>::t zld(rsrc_t r, unsigned off) {
   return __builtin_bit_cast(typename ZVec<T>::t, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
 }
+// Output stores carry the non-temporal hint (cache policy bit 1, "nt" on gfx950): H psi is
+// written once and not read again by this launch, so it should not displace the geometry
+// and the psi halo rows that the neighbouring tiles and orbital groups re-read from the
+// caches (c2 FP32 block: 51.7 -> 49.6 us, identical results).  Non-temporal psi loads
+// were measured slower (56.6 us): the halo rows need the cache.
+constexpr int ZV_STORE_NT = 2;
 template <typename T>
 __device__ __forceinline__ void zst(rsrc_t r, unsigned off, typename ZVec<T>::t v) {
-  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, (int)off, 0, 0);
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, (int)off, 0, ZV_STORE_NT);
 }
 // Value of the lane one to the left (CTRL = 0x111, row_shr:1) or right (0x101,
 // row_shl:1) inside its 16-lane row; lanes without a source get 0 (replaced by the
