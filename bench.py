@@ -16,6 +16,8 @@ Prints one JSON line:
   eigensolve_poisson_benzene  the metric's second part on its own configuration: benzene
                               on the 224^3 monitor grid, k = 48, eigensolve + Poisson in
                               FP64 and in FP32 + FP64 refinement, and the adjoint forces
+  products_mfma               TFLOP/s and MFMA fraction of the eigensolver's rotation and
+                              Gram products at that size, FP32 and FP64
 Algorithmic bytes per block apply: B = N (2 s_psi k + 7 s_g + s_V).
 """
 
@@ -119,6 +121,7 @@ def solver_leg(geom, dg, cfg, veff64, eig_tol, order=2, block=None):
         torch.cuda.synchronize()
         out[f"poisson_{prec}_s"] = round(time.perf_counter() - t0, 4)
         out[f"poisson_{prec}_iterations"] = sol["iterations"]
+        out[f"poisson_{prec}_solves_per_s"] = round(1.0 / max(out[f"poisson_{prec}_s"], 1e-9), 2)
         pots[prec] = sol["potential"]
         print(f"[bench] poisson {prec}: {out[f'poisson_{prec}_s']} s", file=sys.stderr, flush=True)
     out["poisson_mixed_vs_fp64_max_rel"] = float((pots["mixed"] - pots["fp64"]).abs().max()
@@ -159,6 +162,48 @@ def solver_leg(geom, dg, cfg, veff64, eig_tol, order=2, block=None):
     out["workload"] = (f"{dg.shape[0]}^3 grid, Poisson: {cfg.n_charges} Gaussian charges, order-{order} multipoles, "
                        f"rtol 1e-8; eigensolve: k={k}, working subspace {m}, residual tol {eig_tol}, "
                        "trial_fix closure, V = two Z=3 soft-Coulomb centres + v_H")
+    return out
+
+
+MFMA_PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}
+
+
+def products_leg(n=224, m=192, k=48, reps=3):
+    """MFMA utilisation of the eigensolver's two product shapes at the benzene block size
+    (N = n^3, working subspace m, k states), timed with HIP events on the launch stream:
+    the basis rotation Z' = Q [Z; R] (2 m (m + k) N flop, the panelled library GEMM) and
+    the Rayleigh-Ritz Gram block [Z; R] [R; HR]^T (2 (m + k) 2k N flop, split-K batched
+    GEMM with FP64 partial sums), in FP32 and FP64, on seeded random blocks."""
+    import torch
+
+    from isogriddft_amd.eigensolver import _gemm_nt_split, _mm
+
+    N = n ** 3
+    out = {"shapes": f"N = {n}^3, rotation ({m} x {m + k}) x ({m + k} x N), Gram ({m + k} x N) x (N x {2 * k})"}
+    for name, dt in (("fp32", torch.float32), ("fp64", torch.float64)):
+        g = torch.Generator(device="cuda")
+        g.manual_seed(5)
+        B = torch.randn((m + k, N), dtype=dt, device="cuda", generator=g)
+        Q = torch.randn((m, m + k), dtype=dt, device="cuda", generator=g)
+        C = torch.empty((m, N), dtype=dt, device="cuda")
+        legs = (("rotation", lambda: _mm(Q, B, out=C), 2.0 * m * (m + k) * N),
+                ("gram", lambda: _gemm_nt_split(B, B[:2 * k]), 2.0 * (m + k) * 2 * k * N))
+        for what, fn, flop in legs:
+            fn()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            tf = flop / (ms * 1e-3) / 1e12
+            out[f"{what}_{name}"] = {"ms": round(ms, 3), "tflops": round(tf, 2), "peak": MFMA_PEAK_TFLOPS[name],
+                                     "frac": round(tf / MFMA_PEAK_TFLOPS[name], 4)}
+        del B, Q, C
+        torch.cuda.empty_cache()
+    print(f"[bench] products: {out}", file=sys.stderr, flush=True)
     return out
 
 
@@ -298,6 +343,8 @@ def main():
         if args.molecule != "none":
             torch.cuda.empty_cache()
             line[f"eigensolve_poisson_{args.molecule}"] = molecule_leg(args.molecule)
+            torch.cuda.empty_cache()
+            line["products_mfma"] = products_leg()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(geom, psi_np, veff64.cpu().numpy().reshape(dg.shape), args.cpu_seconds)
     print(json.dumps(line), flush=True)
