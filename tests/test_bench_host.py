@@ -1,0 +1,27 @@
+"""Host-side checks of the benchmark script: the one-GPU contract and the algorithmic
+byte count the reported bandwidth divides by."""
+
+import subprocess
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parents[1]
+sys.path.insert(0, str(ROOT))
+
+
+def test_bench_rejects_more_than_one_gpu():
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+    assert r.stdout == ""
+
+
+def test_algorithmic_bytes_per_block_apply():
+    import bench
+
+    # 96^3 FP32 block of 16 orbitals: psi read and H psi written once per orbital, the six
+    # flux coefficients and 1/J once, V once: N (2 * 4 * 16 + 7 * 4 + 4) bytes
+    assert bench.algorithmic_bytes(96 ** 3, 16, 4, 4, 4) == 884736 * 160 == 141557760
+    # 224^3 FP64 block of 48 orbitals with FP64 geometry
+    assert bench.algorithmic_bytes(224 ** 3, 48, 8, 8, 8) == 224 ** 3 * (2 * 8 * 48 + 7 * 8 + 8)
