@@ -26,8 +26,8 @@ eigenproblems run on the host (LAPACK).
 The preconditioner is an addition to the reference's block iteration (which has none):
 it changes the iterates, not the Ritz pairs converged to.  The kinetic one cuts the c2
 workload (96^3, k = 16) from 161 iterations with the diagonal preconditioner to 15, and
-makes the 224^3, k = 48 block converge (72 iterations; the diagonal one had not converged
-after 400).  W is the inner product that
+makes the 224^3, k = 48 block converge (the diagonal one had not converged after 400
+iterations).  W is the inner product that
 makes H self-adjoint: the cell volumes on monitor grids; on legacy grids the product
 weights (h s_x)(h s_y)(h s_z) of the 7-point flux form, with the returned orbitals then
 re-orthonormalised under the cell volumes as the reference returns them.
