@@ -68,11 +68,18 @@ def apply_operator(
     nblk = xb.shape[0]
     if out is None:
         out = torch.empty_like(xb)
+    elif (not out.is_cuda or out.dtype != x.dtype or out.numel() != xb.numel() or not out.is_contiguous()):
+        # the kernel writes nblk * N values through a raw pointer: a shorter, strided or
+        # differently typed `out` would be overrun or silently left unwritten
+        raise ValueError(f"out must be a contiguous CUDA tensor of {xb.numel()} {x.dtype} values "
+                         f"(the shape of the input block); received {tuple(out.shape)} {out.dtype}.")
     ob = out.reshape(-1, *shape)
     if ob.data_ptr() == xb.data_ptr():
         raise ValueError("out must not alias the input block")
     if veff is not None:
         veff = veff.reshape(-1)
+        if veff.numel() != dg.N:
+            raise ValueError(f"veff must have {dg.N} points; received {veff.numel()}.")
         if veff.dtype != x.dtype:
             veff = veff.to(x.dtype)
         veff = veff.contiguous()
@@ -96,7 +103,7 @@ def apply_operator(
         _lib.call(f"ig_legacy_stencil_{sfx}", _lib.ptr(xb), _lib.ptr(ob), nblk, dg.N, nx, ny, nz,
                   _lib.ptr(dg.coef(dt)), _lib.ptr(veff), float(scale), int(transpose),
                   int(interior_only), st)
-    return out.reshape(x.shape) if out.numel() == x.numel() else out
+    return out.reshape(x.shape)
 
 
 def _run(psi, grid_geometry, scale, closure):

@@ -73,6 +73,9 @@ def gram(X: torch.Tensor, Y: torch.Tensor, w: torch.Tensor, out: torch.Tensor | 
         raise ValueError("weights must match the orbital grid size.")
     if out is None:
         out = torch.zeros((m, n), dtype=torch.float64, device=X.device)
+    elif out.dtype != torch.float64 or tuple(out.shape) != (m, n) or not out.is_contiguous():
+        raise ValueError(f"out must be a contiguous ({m}, {n}) float64 tensor; received "
+                         f"{tuple(out.shape)} {out.dtype}.")
     lib = _lib.load()
     need = lib.ig_gram_work_size(m, n, N)
     work = workspace(X.device, need, "gram")

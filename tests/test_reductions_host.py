@@ -36,6 +36,11 @@ def _alpha(k=K):
     lambda: red.cg_finish(_blk(), _blk(), _vec(N - 2), _alpha(), _blk()),
     lambda: red.cg_finish(_blk(), _blk(), _vec(), _alpha(), _blk(torch.float32)),
     lambda: red.cg_finish(_blk(), _blk(), _vec(), _alpha(2), _blk()),
+    lambda: red.gram(_blk(), _blk(torch.float32), None),                    # dtype mismatch
+    lambda: red.gram(_blk(), torch.zeros((2, N + 1), dtype=torch.float64), None),
+    lambda: red.gram(_blk(), _blk(), _vec(N - 1)),
+    lambda: red.gram(_blk(), _blk(), None, out=torch.zeros((K, K - 1), dtype=torch.float64)),
+    lambda: red.gram(_blk(), _blk(), None, out=torch.zeros((K, K), dtype=torch.float32)),
 ])
 def test_malformed_calls_are_refused(call):
     with pytest.raises(ValueError) as e:
