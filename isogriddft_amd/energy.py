@@ -63,12 +63,17 @@ def evaluate_ion_ion_repulsion(case=H2_BENCHMARK_CASE) -> float:
 
 def density_device(dg: DeviceGrid, orbitals: torch.Tensor, occupations, out=None, accumulate=False, stream=None):
     """rho = sum_m occ_m psi_m^2 (FP64, flat) for an orbital-major block."""
+    if orbitals.dtype not in (torch.float32, torch.float64):
+        raise ValueError(f"orbitals must be float32 or float64; received {orbitals.dtype}.")
     blk = orbitals.reshape(-1, dg.N)
     occ = torch.as_tensor(np.asarray(occupations, dtype=np.float64)).to(dg.device)
     n = occ.numel()
     if out is None:
         out = torch.zeros(dg.N, dtype=torch.float64, device=dg.device)
         accumulate = False
+    elif out.dtype != torch.float64 or out.numel() != dg.N or not out.is_contiguous():
+        raise ValueError(f"out must be a contiguous float64 field of {dg.N} points; received "
+                         f"{tuple(out.shape)} {out.dtype}.")
     if n == 0:
         if not accumulate:
             out.zero_()

@@ -381,6 +381,13 @@ def _moments_with_frac(dg, rho, order, frac, stream=None):
     return out
 
 
+def _device_rho(dg, rho):
+    """rho as a flat contiguous FP64 CUDA field of the grid, or ValueError."""
+    if not isinstance(rho, torch.Tensor) or not rho.is_cuda or rho.dtype != torch.float64 or rho.numel() != dg.N:
+        raise ValueError(f"rho must be a float64 CUDA field of {dg.N} points.")
+    return rho.reshape(-1).contiguous()
+
+
 def multipole_boundary_device(dg, rho, order=2, mode="corrected_moments", stream=None):
     """Boundary field b (zero inside) and the flat raw moments used, on the device.
 
@@ -391,7 +398,7 @@ def multipole_boundary_device(dg, rho, order=2, mode="corrected_moments", stream
         raise ValueError(f"multipole_order must be 0-4; received {order}.")
     if mode not in _MODES:
         raise ValueError(f"monitor_boundary_construction_mode must be one of {_MODES}; received `{mode}`.")
-    rho = rho.reshape(-1)
+    rho = _device_rho(dg, rho)
     base = _moments_with_frac(dg, rho, order, None, stream)
     moments = base
     split = mode == "legacy_split" and dg.kind == "monitor"
@@ -558,7 +565,7 @@ def solve_poisson_device(dg, rho, *, multipole_order=2, tolerance=1e-8, max_iter
     if precision not in ("fp64", "mixed"):
         raise ValueError(f"precision must be `fp64` or `mixed`; received `{precision}`.")
     pdata = poisson_data(dg)
-    rho = rho.reshape(-1)
+    rho = _device_rho(dg, rho)
     b, moments, base = multipole_boundary_device(dg, rho, multipole_order, mode, stream)
     # rhs = 4 pi rho + L b on interior nodes (L b = -(-L b))
     rhs = torch.empty_like(rho)

@@ -573,7 +573,11 @@ class NonlocalPlan:
     def project(self, psi, stream=None):
         """[(atom, coef (k, nproj), d = coef H^T (k, nproj))] for a block psi (k, *shape)."""
         dg = self.dg
+        if psi.dtype not in (torch.float32, torch.float64) or psi.numel() % dg.N != 0:
+            raise ValueError(f"psi must be a float32 / float64 block of fields of {dg.N} points.")
         blk = psi.reshape(-1, dg.N)
+        if blk.stride(1) != 1:
+            blk = blk.contiguous()
         k = blk.shape[0]
         sfx = "f64" if blk.dtype == torch.float64 else "f32"
         nx, ny, nz = dg.shape
@@ -594,8 +598,14 @@ class NonlocalPlan:
         dg = self.dg
         if projections is None:
             projections = self.project(psi, stream)
+        if out.dtype not in (torch.float32, torch.float64) or out.numel() % dg.N != 0:
+            raise ValueError(f"out must be a float32 / float64 block of fields of {dg.N} points.")
         ob = out.reshape(-1, dg.N)
+        if ob.stride(1) != 1 or ob.data_ptr() != out.data_ptr():
+            raise ValueError("out is updated in place and must have unit stride along the grid")
         k = ob.shape[0]
+        if any(d.shape[0] != k for _, _, d in projections):
+            raise ValueError("the projections do not belong to a block of the size of out")
         sfx = "f64" if ob.dtype == torch.float64 else "f32"
         nx, ny, nz = dg.shape
         for ap, _, d in projections:
