@@ -95,7 +95,8 @@ def cpu_baseline(geom, psi_np, veff_np, seconds):
         "cores": 1,
         "kind": "port",
         "sample": f"FP64 NumPy oracle (ops/kinetic.py restatement), {done} orbital applies "
-                  f"({blocks:.2f} blocks of {k}) on the 96^3 grid in {dt:.1f} s, single thread",
+                  f"({blocks:.2f} blocks of {k}) on the {'x'.join(map(str, psi_np.shape[1:]))} grid in "
+                  f"{dt:.1f} s, single thread",
     }
 
 

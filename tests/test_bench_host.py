@@ -25,3 +25,21 @@ def test_algorithmic_bytes_per_block_apply():
     assert bench.algorithmic_bytes(96 ** 3, 16, 4, 4, 4) == 884736 * 160 == 141557760
     # 224^3 FP64 block of 48 orbitals with FP64 geometry
     assert bench.algorithmic_bytes(224 ** 3, 48, 8, 8, 8) == 224 ** 3 * (2 * 8 * 48 + 7 * 8 + 8)
+
+
+def test_cpu_baseline_leg_on_a_small_grid():
+    """The CPU-baseline leg (the FP64 oracle block apply on the host) reports the fields
+    of the bench contract, with the bandwidth computed from the same byte count."""
+    import numpy as np
+
+    import bench
+    from isogriddft_amd import configs
+
+    geom = configs.build_synthetic_geometry(shape=(12, 13, 14))
+    rng = np.random.default_rng(1)
+    psi = rng.standard_normal((2, 12, 13, 14))
+    veff = rng.standard_normal((12, 13, 14))
+    r = bench.cpu_baseline(geom, psi, veff, 0.05)
+    assert set(r) == {"value", "unit", "cores", "kind", "sample"}
+    assert r["unit"] == "GB/s" and r["cores"] == 1 and r["kind"] == "port" and r["value"] > 0
+    assert "12x13x14 grid" in r["sample"]
