@@ -363,6 +363,18 @@ int ig_green_identity_f64(const double* f, const double* tf, const double* coef,
                           const double* w, int nx, int ny, int nz, double hx, double hy, double hz, int closure,
                           double* work, double* out, hipStream_t stream);
 
+/* ------------------------------------------------------------------ calibration */
+
+/* Streaming kernels that measure the HBM bandwidth a plain 128-bit copy / read gets on
+ * this GPU (the practical ceiling quoted next to the stencil's) and scale the PMC byte
+ * counters.  No reference counterpart: the reference has no GPU code.
+ * ig_stream_copy_f32: out[i] = in[i], i < n, n a multiple of 4, 16-byte aligned.
+ * ig_stream_read_f32: partial[b] = sum over workgroup b's range of in; partial holds
+ * ig_stream_blocks(n) doubles. */
+int ig_stream_blocks(long long n);
+int ig_stream_copy_f32(const float* in, float* out, long long n, hipStream_t stream);
+int ig_stream_read_f32(const float* in, long long n, double* partial, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -82,6 +82,9 @@ SIGNATURES = {
     "ig_cg_finish_f32": [P, P, P, P, P, L, I, L, L, P],
     "ig_green_identity_work_size": [],
     "ig_green_identity_f64": [P, P, P, P, P, I, I, I, D, D, D, I, P, P, P],
+    "ig_stream_blocks": [L],
+    "ig_stream_copy_f32": [P, P, L, P],
+    "ig_stream_read_f32": [P, L, P, P],
 }
 _RESTYPE = {"ig_gram_work_size": L, "ig_knn_work_bytes": L}
 

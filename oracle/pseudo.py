@@ -8,8 +8,12 @@ Restates:
 
 Pinning: V_loc by the closed form with ``math.erf`` (independent scalar loop) and the
 reference-held <V_loc> = -2.173678262801 Ha on the default legacy 51^3 grid; the
-nonlocal term by a brute-force dense P h P^T W matrix; the patch correction by a
-reference-generated fixture (cKDTree neighbour order is only pinned by import).
+nonlocal term (tests/test_oracle_nonlocal.py) by the closed-form projector normalisation
+int r^2 p^2 dr = 1, the analytic overlaps of the s and p projectors with atom-centred
+Gaussians, and the invariants of the dense nonlocal matrix built column by column on tiny
+grids (W-self-adjoint, rank = number of projectors, nonzero eigenvalue h <p, p>_W with
+eigenvector p, additivity over atoms); the patch correction by a reference-generated
+fixture (cKDTree neighbour order is only pinned by import).
 """
 
 from __future__ import annotations
