@@ -95,6 +95,9 @@ template <typename T>
 __global__ __launch_bounds__(IG_THREADS) void bicg_p(long long n, const T* __restrict__ r, T* __restrict__ pv,
                                                      const T* __restrict__ v, const T* __restrict__ dinv,
                                                      T* __restrict__ phat, const double* __restrict__ sc) {
+  // after a breakdown (flag set by an earlier step of the stream) every update is a no-op:
+  // the iterate stays the last finite one
+  if (sc[8] != 0.0) return;
   const double beta = (sc[0] / sc[1]) * (sc[2] / sc[3]);
   const double omega = sc[3];
   for (long long t = (long long)blockIdx.x * IG_THREADS + threadIdx.x; t < n;
@@ -110,11 +113,16 @@ template <typename T>
 __global__ __launch_bounds__(IG_THREADS) void bicg_s(long long n, const T* __restrict__ r, const T* __restrict__ v,
                                                      const T* __restrict__ dinv, T* __restrict__ sv,
                                                      T* __restrict__ shat, double* __restrict__ sc) {
-  const double alpha = sc[0] / sc[4];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    sc[2] = alpha;
-    if (!(fabs(sc[4]) > 1e-300) || !(fabs(sc[0]) > 1e-300)) sc[8] = 1.0;
+  if (sc[8] != 0.0) return;
+  // every block evaluates the same breakdown test, so none writes s from a non-finite alpha
+  const bool broken = !(fabs(sc[4]) > 1e-300) || !(fabs(sc[0]) > 1e-300);
+  const double alpha = broken ? 0.0 : sc[0] / sc[4];
+  if (broken) {
+    __syncthreads();  // every lane of block 0 has read sc[8] before it is set
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc[8] = 1.0;
+    return;
   }
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc[2] = alpha;
   for (long long t = (long long)blockIdx.x * IG_THREADS + threadIdx.x; t < n;
        t += (long long)gridDim.x * IG_THREADS) {
     const T sn = r[t] - T(alpha) * v[t];
@@ -132,22 +140,34 @@ __global__ __launch_bounds__(IG_THREADS) void bicg_x(long long n, T* __restrict_
                                                      const T* __restrict__ rhat, double* __restrict__ sc,
                                                      double* __restrict__ partial) {
   __shared__ double smem[4 * 2];
+  // After a breakdown of an earlier step (flag set) x and r keep their last values and the
+  // partials re-measure that r, so the host reads the residual of the iterate it gets back.
+  // (t, t) = 0 here (t = A shat vanishes, e.g. s = 0 when the rhs spans an invariant
+  // subspace) still takes x += alpha phat, r = s, then raises the flag: the next steps,
+  // whose beta would divide by omega = 0, are no-ops.  Every block sees the same scalars.
+  const bool frozen = sc[8] != 0.0;
+  const bool broken = frozen || !(sc[6] > 0.0);
   const double alpha = sc[2];
-  const double omega = (sc[6] > 0.0) ? sc[5] / sc[6] : 0.0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    sc[3] = omega;
-    if (!(sc[6] > 0.0)) sc[8] = 1.0;
-  }
+  const double omega = broken ? 0.0 : sc[5] / sc[6];
   double acc[2] = {0.0, 0.0};
   for (long long t = (long long)blockIdx.x * IG_THREADS + threadIdx.x; t < n;
        t += (long long)gridDim.x * IG_THREADS) {
-    x[t] += T(alpha) * phat[t] + T(omega) * shat[t];
-    const T rn = sv[t] - T(omega) * tv[t];
-    r[t] = rn;
+    T rn;
+    if (frozen) {
+      rn = r[t];
+    } else {
+      x[t] += T(alpha) * phat[t] + T(omega) * shat[t];
+      rn = sv[t] - T(omega) * tv[t];
+      r[t] = rn;
+    }
     acc[0] += (double)rhat[t] * (double)rn;
     acc[1] += (double)rn * (double)rn;
   }
-  ig_block_sum<2>(acc, smem);
+  ig_block_sum<2>(acc, smem);  // its barriers order every lane's read of sc[8] before the write
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (broken) sc[8] = 1.0;
+    else sc[3] = omega;
+  }
   if (threadIdx.x == 0) {
     partial[blockIdx.x * 2 + 0] = acc[0];
     partial[blockIdx.x * 2 + 1] = acc[1];

@@ -785,7 +785,11 @@ int ig_vloc_gth_f64(const double* xyz, long long N, int natoms, const double* at
 int ig_vloc_gth_grad_f64(const double* xyz, const double* q, long long N, int natoms, const double* atoms,
                          double* work, double* out, hipStream_t s) {
   if (!xyz || !q || !atoms || !out || !work || N <= 0 || natoms <= 0 || natoms > 64) return IG_EINVAL;
-  const int nb = red_blocks(N) > 256 ? 256 : red_blocks(N);
+  // partials: nb x 3 natoms doubles, which must fit the ig_physics_work_size buffer
+  // (1024 x 35 of its doubles): 64 atoms allow 186 blocks
+  int nb = red_blocks(N) > 256 ? 256 : red_blocks(N);
+  const int cap = (1024 * 35) / (3 * natoms);
+  if (nb > cap) nb = cap;
   hipLaunchKernelGGL(vloc_grad_kernel, dim3(nb), dim3(IG_THREADS), 0, s, xyz, q, N, natoms, (const GthAtom*)atoms,
                      work);
   hipLaunchKernelGGL(sum_partials, dim3(1), dim3(256), 0, s, work, nb, 3 * natoms, out);

@@ -607,9 +607,12 @@ def solve_poisson_device(dg, rho, *, multipole_order=2, tolerance=1e-8, max_iter
             u.add_(d32.double(), alpha=scale)
             _matvec(dg, u, tmp, stream)
             torch.sub(rhs, tmp, out=r64)
+            prev = rel if outer > 1 else math.inf
             rel = float(torch.linalg.vector_norm(r64).item()) / rn
             conv = rel <= tolerance
-            if outer > 64:
+            # a correction that does not reduce the FP64 residual (an inner breakdown or
+            # stagnation at the FP32 floor) ends the loop; so does a non-finite residual
+            if outer > 64 or not math.isfinite(rel) or (not conv and rel >= prev):
                 break
         method = "hip_bicgstab_mixed_fp32_fp64" + ("_monitor" if dg.kind == "monitor" else "")
     potential = b + u

@@ -389,17 +389,21 @@ class PatchPlan:
         return out, tuple(scalings)
 
 
-_PATCH_PLANS: dict = {}
-
-
 def patch_plan(dg: DeviceGrid, case, grid_geometry, params: LocalPotentialPatchParameters) -> PatchPlan:
+    """The grid's patch plan for (atoms, patch radii, parameters), built once and cached on
+    the DeviceGrid itself: the kNN indices index that grid's nodes, so the plan lives and
+    dies with it (a cache keyed by object identity could hand a recycled id a stale plan)."""
     atoms = case_atoms(case)
     radii = geometry_patch_radii(grid_geometry)
-    key = (id(dg), tuple(atoms), radii, params)
-    plan = _PATCH_PLANS.get(key)
+    key = (tuple(atoms), radii, params)
+    cache = getattr(dg, "_patch_plans", None)
+    if cache is None:
+        cache = {}
+        dg._patch_plans = cache
+    plan = cache.get(key)
     if plan is None:
         plan = PatchPlan(dg, atoms, radii, params)
-        _PATCH_PLANS[key] = plan
+        cache[key] = plan
     return plan
 
 

@@ -1,0 +1,77 @@
+"""BiCGSTAB breakdown handling of the fused Krylov kernels (csrc/krylov.hip), driven
+through the C ABI with the scalars set from the host (no GPU fault or hang is involved:
+every kernel runs to completion on finite data).
+
+* (rhat, v) = 0 in the s step: the breakdown flag sc[8] is raised and s is not written;
+  the following x and p steps leave x, r and p as they were, and (r, r) is re-measured.
+* (t, t) = 0 in the x step (t = A shat vanishes): x += alpha phat and r = s are still taken
+  (SciPy's early exit on a vanishing s does the same), then the flag is raised.
+Bit-exact checks on the untouched buffers; 1e-15 relative on the updated ones.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from isogriddft_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+N = 5000
+
+
+def _bufs(seed):
+    g = torch.Generator().manual_seed(seed)
+    mk = lambda: torch.randn(N, dtype=torch.float64, generator=g).cuda()  # noqa: E731
+    return {k: mk() for k in ("x", "r", "rhat", "p", "v", "phat", "s", "shat", "t")}
+
+
+def _call(name, *args):
+    _lib.call(name, *args, _lib.stream_handle())
+
+
+def test_s_step_breakdown_freezes_the_iterate():
+    b = _bufs(1)
+    keep = {k: v.clone() for k, v in b.items()}
+    sc = torch.zeros(16, dtype=torch.float64, device="cuda")
+    sc[:9] = torch.tensor([1.0, 1.0, 0.5, 0.25, 0.0, 0.0, 0.0, 3.0, 0.0], dtype=torch.float64)
+    work = torch.zeros(_lib.load().ig_reduction_work_size(N), dtype=torch.float64, device="cuda")
+    P = _lib.ptr
+    _call("ig_bicgstab_s_f64", N, P(b["r"]), P(b["v"]), None, P(b["s"]), P(b["shat"]), P(sc))
+    torch.cuda.synchronize()
+    assert sc[8].item() == 1.0
+    assert torch.equal(b["s"], keep["s"]) and torch.equal(b["shat"], keep["shat"])
+    _call("ig_bicgstab_x_f64", N, P(b["x"]), P(b["phat"]), P(b["shat"]), P(b["s"]), P(b["t"]), P(b["r"]),
+          P(b["rhat"]), P(sc), P(work))
+    _call("ig_bicgstab_p_f64", N, P(b["r"]), P(b["p"]), P(b["v"]), None, P(b["phat"]), P(sc))
+    torch.cuda.synchronize()
+    for k in ("x", "r", "p", "phat"):
+        assert torch.equal(b[k], keep[k]), k
+    rr = float((keep["r"] * keep["r"]).sum())
+    assert abs(sc[7].item() - rr) <= 1e-13 * rr
+    assert abs(sc[0].item() - float((keep["rhat"] * keep["r"]).sum())) <= 1e-12 * rr
+    assert np.isfinite(sc[:9].cpu().numpy()).all()
+
+
+def test_x_step_with_vanishing_t_takes_the_update_then_flags():
+    b = _bufs(2)
+    keep = {k: v.clone() for k, v in b.items()}
+    alpha = 0.37
+    sc = torch.zeros(16, dtype=torch.float64, device="cuda")
+    sc[:9] = torch.tensor([1.0, 1.0, alpha, 0.25, 2.0, 0.0, 0.0, 3.0, 0.0], dtype=torch.float64)
+    work = torch.zeros(_lib.load().ig_reduction_work_size(N), dtype=torch.float64, device="cuda")
+    P = _lib.ptr
+    _call("ig_bicgstab_x_f64", N, P(b["x"]), P(b["phat"]), P(b["shat"]), P(b["s"]), P(b["t"]), P(b["r"]),
+          P(b["rhat"]), P(sc), P(work))
+    torch.cuda.synchronize()
+    assert sc[8].item() == 1.0
+    xr = keep["x"] + alpha * keep["phat"]
+    assert float((b["x"] - xr).abs().max() / xr.abs().max()) <= 1e-15
+    assert torch.equal(b["r"], keep["s"])
+    # the next iteration's p and s steps are no-ops (beta would divide by omega = 0)
+    _call("ig_bicgstab_p_f64", N, P(b["r"]), P(b["p"]), P(b["v"]), None, P(b["phat"]), P(sc))
+    _call("ig_bicgstab_s_f64", N, P(b["r"]), P(b["v"]), None, P(b["s"]), P(b["shat"]), P(sc))
+    torch.cuda.synchronize()
+    for k in ("p", "phat", "s", "shat"):
+        assert torch.equal(b[k], keep[k]), k
+    assert np.isfinite(sc[:9].cpu().numpy()).all()

@@ -75,6 +75,28 @@ def test_local_potential_gradient_matches_oracle():
         assert np.max(np.abs(g - ref)) <= TOL * np.max(np.abs(ref)), (name, g, ref)
 
 
+def test_local_potential_gradient_64_atoms_on_64_cubed():
+    """The largest accepted call: 64 atoms on a 64^3 grid (the per-block partials of 3 x 64
+    values must fit the fixed reduction workspace), against the oracle, 1e-12."""
+    geom = configs.build_synthetic_geometry(shape=(64, 64, 64))
+    dg = DeviceGrid(geom)
+    rng = np.random.default_rng(7)
+    els = ("C", "N", "O", "H")
+    atoms = [(els[i % 4], tuple(float(v) for v in rng.uniform(-3.0, 3.0, 3))) for i in range(64)]
+    rho = np.exp(-0.2 * (geom.x_points ** 2 + geom.y_points ** 2 + geom.z_points ** 2))
+    q = rho * geom.cell_volumes
+    g = pseudo.local_potential_gradient_device(dg, atoms, torch.from_numpy(q).cuda())
+    ref = np.zeros_like(g)
+    for a, (el, pos) in enumerate(atoms):
+        dx, dy, dz = geom.x_points - pos[0], geom.y_points - pos[1], geom.z_points - pos[2]
+        r = np.sqrt(dx * dx + dy * dy + dz * dz)
+        dv = opseudo.local_potential_radial_derivative(r, opseudo.GTH_PADE[el])
+        with np.errstate(divide="ignore", invalid="ignore"):
+            f = np.where(r > 0, q * dv / r, 0.0)
+        ref[a] = [-np.sum(f * dx), -np.sum(f * dy), -np.sum(f * dz)]
+    assert np.max(np.abs(g - ref)) <= TOL * np.max(np.abs(ref))
+
+
 def _orbitals(geom, k, seed):
     rng = np.random.default_rng(seed)
     out = []

@@ -27,3 +27,28 @@ def test_patch_radii_lookup():
     assert pseudo.geometry_patch_radii(SimpleNamespace(patch_radii=(0.5, 0.75))) == (0.5, 0.75)
     with pytest.raises(ValueError):
         pseudo.geometry_patch_radii(SimpleNamespace(patch_radii=()))
+
+
+def test_patch_plan_lives_on_its_grid(monkeypatch):
+    """The kNN plan indexes one grid's nodes: it is cached on that DeviceGrid, never
+    handed to another grid object (not even one that reuses a freed grid's id)."""
+    built = []
+
+    class StubPlan:
+        def __init__(self, dg, atoms, radii, params):
+            built.append(dg)
+            self.dg = dg
+
+    monkeypatch.setattr(pseudo, "PatchPlan", StubPlan)
+    geom = SimpleNamespace(patch_radii=(0.5, 0.5))
+    case = [("H", (0.0, 0.0, -0.7)), ("H", (0.0, 0.0, 0.7))]
+    params = pseudo.LocalPotentialPatchParameters()
+    dg1 = SimpleNamespace()
+    p1 = pseudo.patch_plan(dg1, case, geom, params)
+    assert pseudo.patch_plan(dg1, case, geom, params) is p1 and len(built) == 1
+    del dg1, p1
+    for _ in range(4):  # fresh grids with the same atoms, radii and parameters
+        dg = SimpleNamespace()
+        assert pseudo.patch_plan(dg, case, geom, params).dg is dg
+        del dg
+    assert len(built) == 5
