@@ -79,6 +79,17 @@ int ig_dots_f32(int nd, const float* a0, const float* b0, const float* a1, const
                 const float* a2, const float* b2, const float* w, long long n, double* work,
                 double* out, int o0, int o1, int o2, hipStream_t stream);
 
+/* ig_dots_f64 without a workspace argument and without any allocation: the block partials
+ * go to a 24 KiB file-scope device array of the library (one per GPU, kept until the process
+ * ends, its address looked up once per device on the first call there).  Any n is accepted
+ * (at most 1024 blocks x 3 partials).  Calls that overlap on one GPU (concurrent streams or
+ * host threads) share that array and race; serialise them.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight loops, harnesses that forbid hipMalloc). */
+int ig_dots_noalloc_f64(int nd, const double* a0, const double* b0, const double* a1, const double* b1,
+                        const double* a2, const double* b2, const double* w, long long n, double* out,
+                        int o0, int o1, int o2, hipStream_t stream);
+
 /* Fused preconditioned-BiCGSTAB vector updates with device-resident scalars sc[9]
  * (rho, rho_prev, alpha, omega, (rhat,v), (t,s), (t,t), (r,r), breakdown flag);
  * dinv = Jacobi preconditioner (NULL = none).

@@ -28,6 +28,7 @@ SIGNATURES = {
     "ig_reduction_work_size": [L],
     "ig_dots_f64": [I, P, P, P, P, P, P, P, L, P, P, I, I, I, P],
     "ig_dots_f32": [I, P, P, P, P, P, P, P, L, P, P, I, I, I, P],
+    "ig_dots_noalloc_f64": [I, P, P, P, P, P, P, P, L, P, I, I, I, P],
     "ig_bicgstab_p_f64": [L, P, P, P, P, P, P, P],
     "ig_bicgstab_p_f32": [L, P, P, P, P, P, P, P],
     "ig_bicgstab_s_f64": [L, P, P, P, P, P, P, P],

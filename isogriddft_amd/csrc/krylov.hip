@@ -10,6 +10,8 @@
 // solution is the same.
 #include "common.h"
 
+#include <mutex>
+
 namespace {
 
 constexpr int RED_BLOCKS_MAX = 1024;
@@ -203,11 +205,51 @@ int bicg_step_x(long long n, T* x, const T* phat, const T* shat, const T* sv, co
   return IG_OK;
 }
 
+// ---- the entry point that allocates nothing: its partial sums go to a file-scope device
+// array instead of a caller workspace.  Each GPU holds its own copy of `scratch` (24 KiB:
+// at most RED_BLOCKS_MAX blocks x 3 partials, every length n fits), whose address is looked
+// up once per device with that device current.  In host code `scratch` names a host-side
+// stand-in, never a device address: only the looked-up pointer is passed to kernels.
+constexpr int NOALLOC_SCRATCH_DOUBLES = RED_BLOCKS_MAX * 3;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+}  // namespace
+
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+std::mutex g_scratch_mutex;
+
+int scratch_for_current_device(double** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) return IG_EHIP;
+  if (dev >= NOALLOC_MAX_DEVICES) return IG_EUNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return IG_EHIP;
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return IG_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
 int ig_reduction_work_size(long long n) { return red_blocks(n) * 3; }
+
+int ig_dots_noalloc_f64(int nd, const double* a0, const double* b0, const double* a1, const double* b1,
+                        const double* a2, const double* b2, const double* w, long long n, double* out, int o0,
+                        int o1, int o2, hipStream_t s) {
+  if (nd < 1 || nd > 3 || n <= 0 || !out || !a0 || !b0) return IG_EINVAL;
+  if ((long long)red_blocks(n) * nd > NOALLOC_SCRATCH_DOUBLES) return IG_EINVAL;  // partials must fit
+  double* work = nullptr;
+  const int st = scratch_for_current_device(&work);
+  if (st != IG_OK) return st;
+  return dots_impl<double>(nd, a0, b0, a1, b1, a2, b2, w, n, work, out, o0, o1, o2, 0, s);
+}
 
 int ig_dots_f64(int nd, const double* a0, const double* b0, const double* a1, const double* b1,
                 const double* a2, const double* b2, const double* w, long long n, double* work,
