@@ -58,6 +58,35 @@ def start_density(dg, atoms):
     return rho * (ztot / wdot(dg, rho, None))
 
 
+EDGE_GAP_HA = 1e-3
+
+
+def occupation_edge(eigenvalues, nocc, min_gap=EDGE_GAP_HA):
+    """(n, gap): the occupied-state count nearest to nocc (ties: the lower) whose edge
+    eps[n] - eps[n - 1] is at least min_gap, among 1 <= n < len(eigenvalues); (len, None) if
+    no computed edge is that wide."""
+    ev = np.asarray(eigenvalues, dtype=np.float64)
+    cands = [n for n in range(1, ev.size) if ev[n] - ev[n - 1] >= min_gap]
+    if not cands:
+        return int(ev.size), None
+    n = min(cands, key=lambda c: (abs(c - nocc), c))
+    return int(n), float(ev[n] - ev[n - 1])
+
+
+def energy_components(dg, mol, orbitals, nocc):
+    """T_s, E_loc, E_nl, E_H, E_xc (lda,vwn), E_II and the total (Ha) of the closed-shell
+    occupation (2 electrons in each of the lowest nocc states) of an orbital block, as the
+    reference's evaluate_single_point_energy (scf/driver.py:905-1015) forms them; the
+    trial_fix kinetic operator of the eigensolve."""
+    from .energy import SpinOccupations, energy_components_device
+
+    orbs = orbitals.reshape(-1, dg.N)[:nocc].to(torch.float64).contiguous()
+    occ = SpinOccupations(float(nocc), float(nocc), (1.0,) * nocc, (1.0,) * nocc)
+    e = energy_components_device(dg, orbs, orbs, occ, mol, closure="trial_fix", functional="lda,vwn")
+    return {"T_s": e.kinetic, "E_loc": e.local_ionic, "E_nl": e.nonlocal_ionic, "E_H": e.hartree, "E_xc": e.xc,
+            "E_II": e.ion_ion_repulsion, "E_total": e.total}
+
+
 def run(name="benzene", n=224, k=None, block=None, tol=3e-4, multipole_order=4, grid_device="cuda"):
     mol, half, k_default, nocc = MOLECULES[name]
     k = k or k_default
@@ -121,10 +150,25 @@ def run(name="benzene", n=224, k=None, block=None, tol=3e-4, multipole_order=4, 
         if prec == "fp64":
             res64 = res
         else:
+            resm = res
             out["eig_fp32_iterations"] = res.fp32_iterations
         _log(f"eigensolve {prec}: {out[f'eig_{prec}_s']} s, {res.iterations} iterations")
     out["eigenvalues_fp64_Ha"] = [round(float(e), 10) for e in ev["fp64"]]
+    out["eig_mixed_vs_fp64_abs_Ha"] = [float(abs(a - b)) for a, b in zip(ev["mixed"], ev["fp64"])]
     out["eig_mixed_vs_fp64_max_abs_Ha"] = float(np.max(np.abs(ev["mixed"] - ev["fp64"])))
+    # Energy components of a closed-shell occupation, from the orbitals of each precision.
+    # They are functions of the occupied subspace, which is only defined where the occupation
+    # edge sits in a gap: the lowest states of these fixed potentials come in clusters split by
+    # 1e-10 Ha or less (modes of the discrete operator localised in the GTH wells, one set per
+    # heavy atom), and an edge inside a cluster makes the occupied density arbitrary.  So the
+    # edge is moved to the nearest gap of at least EDGE_GAP_HA; the count used is reported.
+    out["occupation_edge_gap_Ha"] = float(ev["fp64"][nocc] - ev["fp64"][nocc - 1]) if k > nocc else None
+    n_e, gap = occupation_edge(ev["fp64"], nocc)
+    out["energy_occupied_states"] = n_e
+    out["energy_edge_gap_Ha"] = gap
+    comps = {prec: energy_components(dg, mol, r.orbitals, n_e) for prec, r in (("fp64", res64), ("mixed", resm))}
+    out["energy_components_fp64_Ha"] = comps["fp64"]
+    out["energy_mixed_vs_fp64_abs_Ha"] = {key: abs(comps["mixed"][key] - comps["fp64"][key]) for key in comps["fp64"]}
     out["eigensolve_plus_poisson_fp64_s"] = round(out["eig_fp64_s"] + out["poisson_fp64_s"], 3)
     out["eigensolve_plus_poisson_mixed_s"] = round(out["eig_mixed_s"] + out["poisson_mixed_s"], 3)
 

@@ -43,3 +43,24 @@ def test_cpu_baseline_leg_on_a_small_grid():
     assert set(r) == {"value", "unit", "cores", "kind", "sample"}
     assert r["unit"] == "GB/s" and r["cores"] == 1 and r["kind"] == "port" and r["value"] > 0
     assert "12x13x14 grid" in r["sample"]
+
+
+def test_cpu_hpsi_threads_give_the_same_applies():
+    """The threaded CPU leg applies the same oracle operator; its byte count per orbital
+    apply is the one-orbital block count."""
+    import numpy as np
+
+    import bench
+    from isogriddft_amd import configs
+
+    geom = configs.build_synthetic_geometry(shape=(10, 11, 12))
+    rng = np.random.default_rng(2)
+    psi = rng.standard_normal((4, 10, 11, 12))
+    veff = rng.standard_normal((10, 11, 12))
+    r1 = bench.cpu_hpsi(geom, psi, veff, 0.02, threads=1)
+    r2 = bench.cpu_hpsi(geom, psi, veff, 0.02, threads=2)
+    for r in (r1, r2):
+        assert r["orbital_applies"] % 4 == 0 and r["orbital_applies"] >= 4
+        assert abs(r["gbs"] - bench.algorithmic_bytes(1320, 1, 8, 8, 8) * r["orbital_applies_per_s"] / 1e9) < 1e-9
+    threads, nproc = bench.host_threads()
+    assert 1 <= threads <= min(nproc, bench.MAX_HOST_THREADS)

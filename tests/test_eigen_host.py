@@ -52,3 +52,13 @@ def test_split_k_products_on_host_tensors(N):
     buf = torch.randn((9, N + 7), dtype=torch.float64, generator=g)[:, :N]
     got = _gemm_nt_split(buf[:5], buf[3:])
     assert torch.allclose(got, buf[:5] @ buf[3:].T, rtol=1e-13, atol=1e-11)
+
+
+def test_occupation_edge_moves_to_the_nearest_gap():
+    from isogriddft_amd.workloads import occupation_edge
+
+    ev = [-3.7, -3.7 + 1e-10, -3.7 + 2e-10, -3.2, -3.2 + 1e-11, -0.8, -0.5]
+    assert occupation_edge(ev, 4) == (3, ev[3] - ev[2])        # edge 4 splits a cluster; 3 and 5 tie -> 3
+    assert occupation_edge(ev, 5)[0] == 5
+    assert occupation_edge(ev, 6)[0] == 6
+    assert occupation_edge([-1.0, -1.0 + 1e-12], 1) == (2, None)
