@@ -6,9 +6,19 @@ This module therefore restates published closed forms, not the reference:
   * Slater exchange with the spin-scaling relation E_x[a,b] = (E_x[2a] + E_x[2b]) / 2.
   * PW92 correlation, J. P. Perdew and Y. Wang, Phys. Rev. B 45, 13244 (1992), Table I
     parameters with p = 1, interpolation via f(zeta) and f''(0) = 1.709921.
-Parity vs. the reference: unpinned (third-party VWN); pinned instead by the
-paper's closed forms, the uniform-gas limits and by checking the potentials
-v_sigma = d(rho eps)/d rho_sigma against complex-step derivatives of the energy.
+The reference's functional, lda,vwn (Slater + VWN5), is restated below as well.
+
+Pinning (tests/test_oracle_pins.py, tests/test_oracle_math.py):
+  * VWN at zeta = 0 against the reference's recorded libxc values: E_xc = -0.869107874980 Ha
+    and <psi|v_xc|psi> = -0.568636849315 Ha (audit/baselines.py:822, :1069), to 1e-11.
+  * Both functionals: the paramagnetic and ferromagnetic limits in closed form, the
+    potentials v_sigma = d(rho eps)/d rho_sigma by central differences, eps_c even in zeta,
+    the spin stiffness d^2 eps_c / d zeta^2 following the exact high-density law
+    -(1/(6 pi^2)) ln rs, and VWN5 and PW92 agreeing within 1 mHa at intermediate zeta.
+  * Parity unpinned: VWN5 at 0 < |zeta| < 1 against libxc itself.  The reference holds no
+    polarised libxc value and PySCF/libxc is not available here; the checks above fix the
+    interpolation's structure and its leading stiffness, not the fitted subleading
+    stiffness parameters.
 The density validation follows xc/lsda.py:57-67.
 """
 

@@ -204,3 +204,43 @@ def test_vwn_potentials_are_energy_derivatives_and_limits():
     np.testing.assert_allclose(eps1, ex_f + xc._vwn_g(x, xc.VWN_F)[0], rtol=1e-13)
     # the paramagnetic fit reproduces the VWN paper's ec(rs = 1) ~ -0.0600 Ha (Ceperley-Alder)
     assert abs(xc._vwn_g(np.array([1.0]), xc.VWN_P)[0][0] - (-0.0600)) < 5e-4
+
+
+def _eps_c(fn, rs, z):
+    """Correlation energy per electron of an LSDA oracle at (rs, zeta): the total minus the
+    exact Slater exchange of the two spin densities."""
+    rho = 3.0 / (4.0 * np.pi * rs ** 3)
+    a, b = rho * (1.0 + z) / 2.0, rho * (1.0 - z) / 2.0
+    eps = fn(np.atleast_1d(a), np.atleast_1d(b))[0]
+    ex = -0.75 * (6.0 / np.pi) ** (1.0 / 3.0) * (a ** (4.0 / 3.0) + b ** (4.0 / 3.0)) / rho
+    return eps - ex
+
+
+@pytest.mark.parametrize("fn", [xc.lsda_pw92, xc.lsda_vwn])
+def test_spin_stiffness_high_density_limit_and_spin_symmetry(fn):
+    """The spin-interpolation term of both functionals: the spin stiffness
+    alpha_c(rs) = d^2 eps_c / d zeta^2 at zeta = 0 (central differences in zeta) follows the
+    exact high-density (RPA) law alpha_c ~ -(1 / (6 pi^2)) ln rs + const, so its slope in
+    ln rs tends to -1/(6 pi^2); a dropped or sign-flipped stiffness term, or a wrong f''(0),
+    breaks it.  eps_c is even in zeta."""
+    d = 1e-3
+    rss = np.array([1e-5, 1e-4])
+    alpha = [float((_eps_c(fn, rs, d) - 2.0 * _eps_c(fn, rs, 0.0) + _eps_c(fn, rs, -d))[0] / d ** 2) for rs in rss]
+    assert alpha[0] > alpha[1] > 0.0
+    slope = (alpha[1] - alpha[0]) / np.log(10.0)
+    assert abs(slope + 1.0 / (6.0 * np.pi ** 2)) <= 5e-4 / (6.0 * np.pi ** 2), slope
+    for rs in (0.5, 2.0, 7.0):
+        for z in (0.2, 0.55, 0.9):
+            assert abs(_eps_c(fn, rs, z)[0] - _eps_c(fn, rs, -z)[0]) <= 1e-14
+
+
+def test_vwn_and_pw92_spin_interpolations_agree():
+    """VWN5 and PW92 are independent fits of the same uniform-gas correlation energy
+    (Ceperley-Alder and its spin dependence); at intermediate polarisations they agree to
+    better than 1 mHa for 0.5 <= rs <= 10, while the spin-stiffness term itself is several
+    mHa there, so a slip in either interpolation shows."""
+    worst = 0.0
+    for rs in np.linspace(0.5, 10.0, 20):
+        for z in (0.3, 0.6, 0.9):
+            worst = max(worst, abs(_eps_c(xc.lsda_vwn, rs, z)[0] - _eps_c(xc.lsda_pw92, rs, z)[0]))
+    assert worst <= 1e-3, worst
