@@ -889,6 +889,244 @@ int launch_monitor_zv(const MonParams<T>& p, int nb, hipStream_t s) {
 }
 
 
+// ---------------------------------------------------------------------------------
+// Register-queue forward trial_fix stencil: many orbitals per geometry fetch.
+// Same tile (TY x TZ = 8 x 32, 256 lanes), same expression tree and x march as
+// monitor_stencil_kernel, but the x neighbours come from registers: every lane owns one
+// interior point and at most one extra point of the 12 x 36 plane (a y/z halo-flux point,
+// or a point only the current plane needs), and keeps psi of its points for the planes
+// s-1 .. s+2 in a 4-deep register queue (plane s+3 is requested as s-1 retires).  LDS
+// holds only the current plane and the F_y / F_z staging, ~4 KB per orbital instead of
+// ~14 KB, so a workgroup carries NB = 8-12 orbitals of one tile, loads the tile's geometry
+// once for all of them, and two workgroups still fit on a CU.  Two barriers per plane step:
+// after the plane is written (gradients and fluxes read it) and after the fluxes are staged
+// (the divergence reads them); the next plane overwrites the buffers only after both.
+template <typename T, int NB>
+__global__ __launch_bounds__(IG_THREADS, 2) void monitor_rq_kernel(MonParams<T> p) {
+  constexpr int TY = 8, TZ = 32, HP = 2, PY = TY + 2 * HP, PZ = TZ + 2 * HP, PLANE = PY * PZ;
+  constexpr int FYS = (TY + 2) * TZ;      // F_y rows -1 .. TY
+  constexpr int FZS = TY * (TZ + 2);      // F_z columns -1 .. TZ
+  constexpr int NYH = 2 * TZ, NZH = 2 * TY;            // halo-flux points
+  constexpr int NOR = 2 * TZ, NOC = 2 * TY, NCO = 4;   // outer rows, outer columns, corners
+  constexpr int NEX = NYH + NZH + NOR + NOC + NCO;     // lanes with an extra point
+  constexpr unsigned S = sizeof(T);
+  static_assert(TY * TZ == IG_THREADS && NEX <= IG_THREADS, "one interior and at most one extra point per lane");
+
+  // one spare entry each absorbs the writes of lanes without an extra point
+  __shared__ T s_pl[NB][PLANE + 1];
+  __shared__ T s_fl[NB][FYS + FZS + 1];
+
+  const int ngroups = (p.nblk + NB - 1) / NB;
+  const int b = blockIdx.x;
+  const int xcd = b & 7;
+  const int r = b >> 3;
+  const int grp = r % ngroups;
+  const int upx = (p.units + 7) / 8;
+  const int u = xcd * upx + r / ngroups;
+  if (u >= p.units) return;
+  const int tzi = u % p.ntz;
+  const int tyi = (u / p.ntz) % p.nty;
+  const int xci = u / (p.ntz * p.nty);
+  const int nx = p.nx, ny = p.ny, nz = p.nz;
+  const unsigned plane_bytes = (unsigned)(ny * nz) * S;
+  const int j0 = min(tyi * TY, max(ny - TY, 0));
+  const int k0 = min(tzi * TZ, max(nz - TZ, 0));
+  const int x0 = xci * p.xc;
+  const int x1 = min(x0 + p.xc, nx);
+  const long long N = (long long)nx * ny * nz;
+
+  rsrc_t rin[NB], rout[NB];
+  bool orb_ok[NB];
+#pragma unroll
+  for (int n = 0; n < NB; ++n) {
+    const int orb = grp * NB + n;
+    orb_ok[n] = orb < p.nblk;
+    const int oc = orb_ok[n] ? orb : p.nblk - 1;
+    rin[n] = mkbuf(p.in + (long long)oc * p.ld, N);
+    rout[n] = mkbuf(p.out + (long long)oc * p.ld, N);
+  }
+  const rsrc_t rxx = mkbuf(p.coef, N), ryy = mkbuf(p.coef + N, N), rzz = mkbuf(p.coef + 2 * N, N);
+  const rsrc_t rxy = mkbuf(p.coef + 3 * N, N), rxz = mkbuf(p.coef + 4 * N, N), ryz = mkbuf(p.coef + 5 * N, N);
+  const rsrc_t rij = mkbuf(p.invj, N);
+  const rsrc_t rv = mkbuf(p.veff ? p.veff : p.invj, p.veff ? N : 0);
+
+  const int tid = threadIdx.x;
+  auto goff = [&](int row, int col) -> unsigned {  // in-plane byte offset, BAD outside the domain
+    const int j = j0 + row, k = k0 + col;
+    return (j >= 0 && j < ny && k >= 0 && k < nz) ? (unsigned)(j * nz + k) * S : BAD;
+  };
+  // interior point
+  const int ty = tid / TZ, tz = tid % TZ;
+  const int li = (ty + HP) * PZ + tz + HP;
+  const unsigned ioff = goff(ty, tz);
+  // extra point: kind 1 = F_y halo row, 2 = F_z halo column, 0 = plane only, -1 = none
+  int erow = 0, ecol = 0, ekind = -1, est = FYS + FZS;
+  if (tid < NYH) {
+    const int hr = tid / TZ, c = tid % TZ;
+    erow = hr ? TY : -1;
+    ecol = c;
+    ekind = 1;
+    est = (hr ? TY + 1 : 0) * TZ + c;
+  } else if (tid < NYH + NZH) {
+    const int i2 = tid - NYH, side = i2 / TY, rr = i2 % TY;
+    erow = rr;
+    ecol = side ? TZ : -1;
+    ekind = 2;
+    est = FYS + rr * (TZ + 2) + (side ? TZ + 1 : 0);
+  } else if (tid < NYH + NZH + NOR) {
+    const int i3 = tid - NYH - NZH, hr = i3 / TZ;
+    erow = hr ? TY + 1 : -2;
+    ecol = i3 % TZ;
+    ekind = 0;
+  } else if (tid < NYH + NZH + NOR + NOC) {
+    const int i4 = tid - NYH - NZH - NOR, side = i4 / TY;
+    erow = i4 % TY;
+    ecol = side ? TZ + 1 : -2;
+    ekind = 0;
+  } else if (tid < NEX) {
+    const int i5 = tid - NYH - NZH - NOR - NOC;
+    erow = (i5 >> 1) ? TY : -1;
+    ecol = (i5 & 1) ? TZ : -1;
+    ekind = 0;
+  }
+  const int ei = ekind >= 0 ? (erow + HP) * PZ + ecol + HP : PLANE;
+  const unsigned eoff = ekind >= 0 ? goff(erow, ecol) : BAD;
+  const unsigned hoff = ekind >= 1 ? eoff : BAD;  // geometry of a halo-flux point
+  const bool yrow = ekind == 1;
+  const bool has_flux = ekind >= 1;
+  const unsigned toff = (j0 + ty < ny && k0 + tz < nz) ? ioff : BAD;
+
+  auto pbytes = [&](int q) { return (q >= 0 && q < nx) ? (unsigned)q * plane_bytes : BAD; };
+  const T tag = T(0);
+
+  // register queues, slot of plane q: (q - sb + 1) & 3
+  T qi[NB][4], qe[NB][4];
+  T fx[NB][4];   // F_x at the interior point, same slots
+  T dyz[NB][2];  // y/z divergence, slot (q - sb) & 1
+  T gc[2][6], hc[2][3], oij[2], ov[2];  // geometry sets, slot (q - sb) & 1
+  auto fetch_psi = [&](int q, int slot) {
+    const unsigned pb = pbytes(q);
+#pragma unroll
+    for (int n = 0; n < NB; ++n) {
+      qi[n][slot] = bld(rin[n], pb + ioff, tag);
+      qe[n][slot] = bld(rin[n], pb + eoff, tag);
+    }
+  };
+  auto fetch_geom = [&](int q, int w) {
+    const unsigned pb = pbytes(q);
+    const unsigned o = pb + ioff;
+    gc[w][0] = bld(rxx, o, tag); gc[w][1] = bld(ryy, o, tag); gc[w][2] = bld(rzz, o, tag);
+    gc[w][3] = bld(rxy, o, tag); gc[w][4] = bld(rxz, o, tag); gc[w][5] = bld(ryz, o, tag);
+    const unsigned h = pb + hoff;
+    const T a_xy = bld(rxy, h, tag), a_yy = bld(ryy, h, tag), a_yz = bld(ryz, h, tag);
+    const T a_xz = bld(rxz, h, tag), a_zz = bld(rzz, h, tag);
+    hc[w][0] = yrow ? a_xy : a_xz;
+    hc[w][1] = yrow ? a_yy : a_yz;
+    hc[w][2] = yrow ? a_yz : a_zz;
+  };
+  auto fetch_out = [&](int q, int w) {
+    const unsigned o = pbytes(q) + ioff;
+    oij[w] = bld(rij, o, tag);
+    ov[w] = bld(rv, o, tag);
+  };
+
+  const int sb = x0 - 1;  // steps sb .. x1, output plane o = s - 1 in [x0, x1)
+#pragma unroll
+  for (int n = 0; n < NB; ++n) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a) fx[n][a] = T(0);
+    dyz[n][0] = dyz[n][1] = T(0);
+  }
+#pragma unroll
+  for (int d = 0; d < 4; ++d) fetch_psi(sb - 1 + d, d);
+  fetch_geom(sb, 0);
+  fetch_geom(sb + 1, 1);
+  fetch_out(sb - 1, 1);
+  fetch_out(sb, 0);
+  const T sc = p.scale;
+
+  for (int s0 = sb; s0 <= x1; s0 += 4) {
+#pragma unroll
+    for (int U = 0; U < 4; ++U) {
+      const int s = s0 + U;
+      if (s > x1) break;
+      const int sm = U & 3, s0l = (U + 1) & 3, sp = (U + 2) & 3;  // slots of s-1, s, s+1
+      const int W = U & 1;                                         // geometry / dyz set of s
+      // ---- phase A: publish plane s
+#pragma unroll
+      for (int n = 0; n < NB; ++n) {
+        s_pl[n][li] = qi[n][s0l];
+        s_pl[n][ei] = qe[n][s0l];
+      }
+      __syncthreads();
+      // ---- phase B: gradients and fluxes of plane s
+#pragma unroll
+      for (int n = 0; n < NB; ++n) {
+        const T* pl = s_pl[n];
+        {
+          const T gx = qi[n][sp] - qi[n][sm];
+          const T gy = pl[li + PZ] - pl[li - PZ];
+          const T gz = pl[li + 1] - pl[li - 1];
+          fx[n][s0l] = flux3(gc[W][0], gx, gc[W][3], gy, gc[W][4], gz);
+          s_fl[n][(ty + 1) * TZ + tz] = flux3(gc[W][3], gx, gc[W][1], gy, gc[W][5], gz);
+          s_fl[n][FYS + ty * (TZ + 2) + tz + 1] = flux3(gc[W][4], gx, gc[W][5], gy, gc[W][2], gz);
+        }
+        if (has_flux) {
+          const T gx = qe[n][sp] - qe[n][sm];
+          const T gy = pl[ei + PZ] - pl[ei - PZ];
+          const T gz = pl[ei + 1] - pl[ei - 1];
+          s_fl[n][est] = flux3(hc[W][0], gx, hc[W][1], gy, hc[W][2], gz);
+        }
+      }
+      fetch_geom(s + 2, W);
+      __syncthreads();
+      // ---- phase C: y/z divergence of plane s, output plane o = s - 1
+      const int o = s - 1;
+      const unsigned ob = (o >= x0 && o < x1) ? (unsigned)o * plane_bytes : BAD;
+#pragma unroll
+      for (int n = 0; n < NB; ++n) {
+        const int fy0 = (ty + 1) * TZ + tz;
+        const int fz0 = FYS + ty * (TZ + 2) + tz + 1;
+        dyz[n][W] = (s_fl[n][fy0 + TZ] - s_fl[n][fy0 - TZ]) + (s_fl[n][fz0 + 1] - s_fl[n][fz0 - 1]);
+        const T dx = fx[n][s0l] - fx[n][(U + 3) & 3];
+        const T lap = dx + dyz[n][W ^ 1];
+        const T v = fma(ov[W ^ 1], qi[n][sm], sc * (lap * oij[W ^ 1]));
+        bst(rout[n], (orb_ok[n] ? ob : BAD) + toff, v);
+      }
+      fetch_out(s + 1, W ^ 1);
+      fetch_psi(s + 3, sm);
+    }
+  }
+}
+
+// auto_pick: the launcher's own choice, declined (IG_EUNSUPPORTED) when the launch would hold
+// fewer than ~2 residency rounds of workgroups, where the general / z-vectorised kernels
+// fill the GPU better (96^3, 16 orbitals: 0.11-0.21 ms here against 0.052 ms).
+template <typename T, int NB>
+int launch_monitor_rq_t(const MonParams<T>& base, hipStream_t stream, bool auto_pick = false) {
+  MonParams<T> p = base;
+  if ((long long)p.nx * p.ny * p.nz * (long long)sizeof(T) >= (1LL << 30)) return IG_EUNSUPPORTED;
+  if (p.interior_only) return IG_EUNSUPPORTED;
+  p.nty = (p.ny + 7) / 8;
+  p.ntz = (p.nz + 31) / 32;
+  const long long groups = (p.nblk + NB - 1) / NB;
+  if (p.xc <= 0) {
+    // chunks of ~56 planes (the measured best of the 224^3 block for the kernel above)
+    const int c = p.nx >= 112 ? (p.nx + 28) / 56 : 1;
+    p.xc = (p.nx + c - 1) / c;
+  }
+  if (p.xc > p.nx) p.xc = p.nx;
+  p.nxc = (p.nx + p.xc - 1) / p.xc;
+  p.units = p.nty * p.ntz * p.nxc;
+  const long long blocks = 8LL * groups * ((p.units + 7) / 8);
+  if (blocks <= 0 || blocks > 0x7fffffffLL) return IG_EINVAL;
+  if (auto_pick && blocks < 2048) return IG_EUNSUPPORTED;
+  hipLaunchKernelGGL((monitor_rq_kernel<T, NB>), dim3((unsigned)blocks), dim3(IG_THREADS), 0, stream, p);
+  IG_CHECK_LAUNCH();
+  return IG_OK;
+}
+
+
 #define IG_MON_CASE(TY_, TZ_, NB_, PD_)                                                            \
   if (ty == TY_ && tz == TZ_ && nb == NB_ && pd == PD_)                                            \
     return closure ? launch_monitor_t<T, TY_, TZ_, 1, false, NB_, PD_>(p, s)                       \
@@ -901,12 +1139,33 @@ template <typename T>
 int launch_monitor(const MonParams<T>& p, int closure, int transpose, int tile, hipStream_t s) {
   if (transpose) return closure ? launch_monitor_t<T, 8, 32, 1, true>(p, s) : launch_monitor_t<T, 8, 32, 0, true>(p, s);
   if (tile < 0) return IG_EINVAL;
+  if (tile >= 8000000) {
+    if (closure) return IG_EUNSUPPORTED;
+    switch (tile - 8000000) {
+      case 4: return launch_monitor_rq_t<T, 4>(p, s);
+      case 6: return launch_monitor_rq_t<T, 6>(p, s);
+      case 8: return launch_monitor_rq_t<T, 8>(p, s);
+      case 12: return launch_monitor_rq_t<T, 12>(p, s);
+      default: return IG_EUNSUPPORTED;
+    }
+  }
   if (tile >= 7000000) return closure ? IG_EUNSUPPORTED : launch_monitor_zv<T>(p, tile - 7000000, s);
   // FP32 blocks whose geometry stream (8 fields, 32 B a point) outgrows the 256 MB
   // Infinity Cache: four orbitals per workgroup (general kernel) halve the geometry
   // re-reads and beat the z-vectorised kernel's two (224^3, k = 48: 2248 against
   // 2070 GB/s; at 96^3, where the geometry stays cached, 2368 against 2629).
   const bool fp32_big = sizeof(T) == 4 && p.nblk >= 4 && (long long)p.nx * p.ny * p.nz * 32 > (128LL << 20);
+  // Large blocks: the register-queue kernel, whose workgroups carry 6 (FP32) / 4 (FP64)
+  // orbitals with one geometry fetch and still fit 3 / 2 to a CU (224^3, k = 48: FP32
+  // 1.69 ms against 1.90 ms for the general kernel, FP64 3.22 ms against 3.70 ms for the
+  // z-vectorised one).
+  if (tile == 0 && closure == 0 && !transpose && !p.interior_only &&
+      (long long)p.nx * p.ny * p.nz * (long long)sizeof(T) * 8 > (128LL << 20)) {
+    int st = IG_EUNSUPPORTED;
+    if (sizeof(T) == 4 && p.nblk >= 6) st = launch_monitor_rq_t<T, 6>(p, s, true);
+    else if (sizeof(T) == 8 && p.nblk >= 4) st = launch_monitor_rq_t<T, 4>(p, s, true);
+    if (st != IG_EUNSUPPORTED) return st;
+  }
   if (tile == 0 && closure == 0 && !fp32_big) {
     const int st = launch_monitor_zv<T>(p, 0, s);
     if (st != IG_EUNSUPPORTED) return st;
