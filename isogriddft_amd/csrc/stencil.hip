@@ -1001,7 +1001,7 @@ __global__ __launch_bounds__(IG_THREADS, 2) void monitor_rq_kernel(MonParams<T> 
 
   // register queues, slot of plane q: (q - sb + 1) & 3
   T qi[NB][4], qe[NB][4];
-  T fx[NB][4];   // F_x at the interior point, same slots
+  T fx[NB][2];   // F_x at the interior point, slot (q - sb) & 1: F_x(s) replaces F_x(s - 2)
   T dyz[NB][2];  // y/z divergence, slot (q - sb) & 1
   T gc[2][6], hc[2][3], oij[2], ov[2];  // geometry sets, slot (q - sb) & 1
   auto fetch_psi = [&](int q, int slot) {
@@ -1033,8 +1033,7 @@ __global__ __launch_bounds__(IG_THREADS, 2) void monitor_rq_kernel(MonParams<T> 
   const int sb = x0 - 1;  // steps sb .. x1, output plane o = s - 1 in [x0, x1)
 #pragma unroll
   for (int n = 0; n < NB; ++n) {
-#pragma unroll
-    for (int a = 0; a < 4; ++a) fx[n][a] = T(0);
+    fx[n][0] = fx[n][1] = T(0);
     dyz[n][0] = dyz[n][1] = T(0);
   }
 #pragma unroll
@@ -1060,6 +1059,7 @@ __global__ __launch_bounds__(IG_THREADS, 2) void monitor_rq_kernel(MonParams<T> 
       }
       __syncthreads();
       // ---- phase B: gradients and fluxes of plane s
+      T fxs[NB];
 #pragma unroll
       for (int n = 0; n < NB; ++n) {
         const T* pl = s_pl[n];
@@ -1067,7 +1067,7 @@ __global__ __launch_bounds__(IG_THREADS, 2) void monitor_rq_kernel(MonParams<T> 
           const T gx = qi[n][sp] - qi[n][sm];
           const T gy = pl[li + PZ] - pl[li - PZ];
           const T gz = pl[li + 1] - pl[li - 1];
-          fx[n][s0l] = flux3(gc[W][0], gx, gc[W][3], gy, gc[W][4], gz);
+          fxs[n] = flux3(gc[W][0], gx, gc[W][3], gy, gc[W][4], gz);
           s_fl[n][(ty + 1) * TZ + tz] = flux3(gc[W][3], gx, gc[W][1], gy, gc[W][5], gz);
           s_fl[n][FYS + ty * (TZ + 2) + tz + 1] = flux3(gc[W][4], gx, gc[W][5], gy, gc[W][2], gz);
         }
@@ -1088,7 +1088,8 @@ __global__ __launch_bounds__(IG_THREADS, 2) void monitor_rq_kernel(MonParams<T> 
         const int fy0 = (ty + 1) * TZ + tz;
         const int fz0 = FYS + ty * (TZ + 2) + tz + 1;
         dyz[n][W] = (s_fl[n][fy0 + TZ] - s_fl[n][fy0 - TZ]) + (s_fl[n][fz0 + 1] - s_fl[n][fz0 - 1]);
-        const T dx = fx[n][s0l] - fx[n][(U + 3) & 3];
+        const T dx = fxs[n] - fx[n][W];
+        fx[n][W] = fxs[n];
         const T lap = dx + dyz[n][W ^ 1];
         const T v = fma(ov[W ^ 1], qi[n][sm], sc * (lap * oij[W ^ 1]));
         bst(rout[n], (orb_ok[n] ? ob : BAD) + toff, v);
