@@ -268,6 +268,7 @@ def solver_leg(geom, dg, cfg, eig_tol, order=2, block=None):
         out[f"eig_{prec}_converged"] = bool(res.converged)
         ev[prec] = np.asarray(res.eigenvalues)
         out[f"eig_{prec}_phase_s"] = {key: round(v, 4) for key, v in res.timings.items()}
+        out[f"eig_{prec}_active_fraction"] = round(float(np.mean(res.active_rows)) / k, 3) if res.active_rows else 1.0
         print(f"[bench] eigensolve {prec}: {out[f'eig_{prec}_s']} s, {res.iterations} iterations",
               file=sys.stderr, flush=True)
     out["eig_fp32_iterations"] = res.fp32_iterations
@@ -338,7 +339,8 @@ def molecule_leg(name):
     keys = ("workload", "k", "occupied", "grid_s", "poisson_fp64_s", "poisson_mixed_s", "poisson_fp64_iterations",
             "poisson_mixed_iterations", "poisson_mixed_vs_fp64_max_rel", "eig_fp64_s", "eig_mixed_s",
             "eig_fp64_iterations", "eig_mixed_iterations", "eig_fp32_iterations", "eig_fp64_converged",
-            "eig_mixed_converged", "eig_fp64_phase_s", "eig_mixed_phase_s", "eig_mixed_vs_fp64_max_abs_Ha",
+            "eig_mixed_converged", "eig_fp64_phase_s", "eig_mixed_phase_s", "eig_fp64_active_fraction",
+            "eig_mixed_active_fraction", "eig_mixed_vs_fp64_max_abs_Ha",
             "eig_mixed_vs_fp64_abs_Ha", "eigenvalues_fp64_Ha", "occupation_edge_gap_Ha", "energy_occupied_states",
             "energy_edge_gap_Ha", "energy_components_fp64_Ha",
             "energy_mixed_vs_fp64_abs_Ha", "eigensolve_plus_poisson_fp64_s", "eigensolve_plus_poisson_mixed_s",

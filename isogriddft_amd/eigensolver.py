@@ -153,6 +153,12 @@ class BlockIterationResult:
     reprojected_rows: int = 0
     dropped_rows: int = 0
     truncated_steps: int = 0
+    # residual rows that entered the expansion at each iteration (soft locking)
+    active_rows: list = field(default_factory=list)
+    # wall time of each iteration (profiling runs only) and whether the rotations
+    # alternated two buffers (False: column panels rotated in place)
+    iteration_seconds: list = field(default_factory=list)
+    two_buffers: bool = True
 
 
 def _host_geigh(A, B):
@@ -198,6 +204,10 @@ KINETIC_SHIFT = 1.0
 # residual rows whose (twice) projection out of the basis keeps less than DEPENDENT_REL *
 # eps of their norm are treated as lying in the basis span
 DEPENDENT_REL = 10.0
+# soft locking: a residual row below LOCK_FRACTION of the phase's target adds no direction
+# (0.1 keeps the locked states an order of magnitude inside the tolerance, so quantities
+# linear in the orbitals, such as densities and forces, keep most of the unlocked accuracy)
+LOCK_FRACTION = 0.1
 # K panel of the split-K Gram products: about NT_PANELS panels, none shorter than NT_MIN_PANEL
 NT_PANELS = 256
 NT_MIN_PANEL = 4096
@@ -238,7 +248,7 @@ BIG_BUFFER_BYTES = 8 << 30
 
 def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp32_floor=FP32_RESIDUAL_FLOOR,
                              keep_basis=False, profile=False, precondition="kinetic", inplace_rotation=None,
-                             hz_update=None):
+                             hz_update=None, lock=True):
     """Lowest-k eigenpairs of the W-self-adjoint H from the starting block X0 (m, N).
 
     The iteration runs on Z = W^(1/2) X, where the W-inner products become plain ones
@@ -267,6 +277,14 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
     hz_update: how S Z follows the new basis: "rotate" (HZ' = Q [HR; HZ], 2 m (m + k) N
     flop) or "apply" (HZ' = S Z', a bandwidth-bound block apply).  None (default) applies
     once m + k reaches HZ_APPLY_MIN, where the rotation's flops outgrow the apply's bytes.
+
+    lock: soft locking.  A state whose residual norm is at most LOCK_FRACTION of the
+    current phase's target contributes no residual direction to the expansion: its Ritz
+    vector stays in the basis and is still refined by every Rayleigh-Ritz step, its
+    residual is still measured each iteration (a state that drifts back above the
+    threshold is active again), and convergence is still judged on all k residuals.  The
+    preconditioner, the S R apply, the Gram products and the Rayleigh-Ritz size then scale
+    with the active rows only.  False expands with all k residuals every iteration.
     """
     if precision not in ("fp64", "mixed"):
         raise ValueError(f"precision must be `fp64` or `mixed`; received `{precision}`.")
@@ -339,7 +357,7 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
     cg_bufs = {}
 
     def kinetic_cg(Rb):
-        """Rb <- ~ (S_T + c)^-1 Rb row by row.  The CG runs on the X-space rows
+        """Rb <- ~ (S_T + c)^-1 Rb row by row (Rb: at most k rows).  The CG runs on the X-space rows
         x = W^(-1/2) r with the W-self-adjoint T + c and W-inner products (no similarity
         scaling around each stencil apply): KINETIC_CG_STEPS of Jacobi-preconditioned CG
         from x0 = D^-1 r, D = diag(T) + c, then the result is mapped back by W^(1/2)."""
@@ -351,13 +369,14 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
             cg_bufs[dtb] = (torch.empty((4, k, N), dtype=dtb, device=dev), dinv_t, w64.to(dtb), isw_t,
                             isw_t * dinv_t, sw64.to(dtb))
         buf, dinv, wd, iswd, isd, swd = cg_bufs[dtb]
-        x, r, pv, q = buf[0], buf[1], buf[2], buf[3]
+        kb = Rb.shape[0]
+        x, r, pv, q = buf[0, :kb], buf[1, :kb], buf[2, :kb], buf[3, :kb]
         mark("rotation")
         torch.mul(Rb, isd, out=x)                       # x0 = D^-1 W^(-1/2) R
-        kop.apply(x.view((k,) + shape), q.view((k,) + shape))
+        kop.apply(x.view((kb,) + shape), q.view((kb,) + shape))
         rz = cg_start(Rb, iswd, q, dinv, wd, r, pv)     # r = W^(-1/2) R - A x0, p = D^-1 r
         for step in range(KINETIC_CG_STEPS):
-            kop.apply(pv.view((k,) + shape), q.view((k,) + shape))
+            kop.apply(pv.view((kb,) + shape), q.view((kb,) + shape))
             pq = row_dots(pv, q, wd)
             alpha = torch.where(pq > 0, rz / torch.where(pq > 0, pq, 1.0), 0.0)
             if step == KINETIC_CG_STEPS - 1:
@@ -381,10 +400,13 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
         Linv = _chol_inv(G).to(dev)
         rotate(Linv.to(Z.dtype), Z, Z, None)
 
-    # One buffer holds the rows [Z; R; HR; HZ] (basis, residuals, S R, S Z), so [Z; R],
-    # [R; HR] and [HR; HZ] are contiguous: each basis rotation is one GEMM and one pass
-    # of the Gram kernel gives Z R^T, Z HR^T, R R^T and R HR^T together.  Two buffers
-    # alternate between iterations, or one is rotated in place when two do not fit.
+    # One buffer holds the rows [Z; R; HR; HZ] (basis, k residual rows, k rows for S R,
+    # S Z).  The ka active residuals are compacted to the first rows of R and their S R
+    # written right behind them, so [Z; R_a] and [R_a; HR_a] are contiguous: the basis
+    # rotation is one GEMM and one pass of the Gram kernel gives Z R_a^T, Z HR_a^T,
+    # R_a R_a^T and R_a HR_a^T together.  Before the S Z rotation HR_a moves to the last
+    # ka rows of the HR block, so [HR_a; HZ] is contiguous too (no move when ka = k).  Two
+    # buffers alternate between iterations, or one is rotated in place when two do not fit.
     def views(B):
         return B[:m], B[m:m + k], B[m + k:m + 2 * k], B[m + 2 * k:]
 
@@ -442,6 +464,8 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
     for it in range(1, max_iter + 1):
         Z, R, HR, HZ = views(BUF)
         mark("rotation")
+        if profile:
+            res.iteration_seconds.append(clock[0])
         if refresh or it % 16 == 1:
             Gzz = _sym(gemm_nt(Z, Z))
             P = _sym(gemm_nt(Z, HZ))
@@ -506,17 +530,28 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
             res.iterations, res.converged = it, True
             res.ritz_matrix = ritz_k
             break
+        # soft locking: the active residual rows, compacted to the front of R in order
+        # (row act[j] >= j, so no row is overwritten before it is read)
+        act = np.nonzero(rn > LOCK_FRACTION * target)[0] if lock else np.arange(k)
+        if len(act) == 0:
+            act = np.arange(k)
+        ka = len(act)
+        res.active_rows.append(ka)
+        for j, i in enumerate(act):
+            if i != j:
+                R[j].copy_(R[i])
+        Ra, HRa = BUF[m:m + ka], BUF[m + ka:m + 2 * ka]
         mark("rotation")
         if hdiag is not None:
-            den = hdiag[None, :] - torch.from_numpy(th).to(dev)[:, None]
-            R.div_(den.clamp_min_(PRECONDITIONER_FLOOR).to(dt))
+            den = hdiag[None, :] - torch.from_numpy(th[act]).to(dev)[:, None]
+            Ra.div_(den.clamp_min_(PRECONDITIONER_FLOOR).to(dt))
         elif pmode == "kinetic":
-            kinetic_cg(R)
+            kinetic_cg(Ra)
         # project the residuals out of the basis and whiten them
-        GRZ = gemm_nt(R, Z)
+        GRZ = gemm_nt(Ra, Z)
         mark("gram")
-        R.sub_(_mm(GRZ.to(dt), Z))
-        GR = gemm_nt(R, R).cpu()
+        Ra.sub_(_mm(GRZ.to(dt), Z))
+        GR = gemm_nt(Ra, Ra).cpu()
         mark("gram")
         # |R_pre|^2 = |R|^2 + |R Z^T|^2 (Z orthonormal).  Rows that lost more than 99 % of
         # their norm to the projection carry its rounding error: they are projected once
@@ -529,60 +564,67 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
         if bool(again.any()):
             res.reprojected_rows += int(again.sum())
             idx = again.nonzero().flatten().to(dev)
-            Rf = R.index_select(0, idx)
+            Rf = Ra.index_select(0, idx)
             Rf.sub_(_mm(gemm_nt(Rf, Z).to(dt), Z))
-            R.index_copy_(0, idx, Rf)
+            Ra.index_copy_(0, idx, Rf)
             del Rf
-            GR = gemm_nt(R, R).cpu()
+            GR = gemm_nt(Ra, Ra).cpu()
             d = torch.diagonal(GR).clone()
             mark("gram")
         dead = d <= (DEPENDENT_REL * torch.finfo(dt).eps) ** 2 * d_pre
         if bool(dead.any()):
             res.dropped_rows += int(dead.sum())
-            R[dead.to(dev)] = 0
+            Ra[dead.to(dev)] = 0
             GR[dead, :] = 0
             GR[:, dead] = 0
             GR[dead, dead] = 1.0
         Linv_r = _chol_inv(GR).to(dev)
         mark("eigh")
-        R.copy_(_mm(Linv_r.to(dt), R))
-        apply_s(R, HR)
-        G23 = gemm_nt(BUF[:m + k], BUF[m:m + 2 * k]).cpu()  # [Z; R] [R; HR]^T: (m + k, 2k)
-        # R HZ^T: (k, m); equal to (Z HR^T)^T up to rounding when S is symmetric
-        G4 = None if symmetric else gemm_nt(R, HZ).cpu()
+        Ra.copy_(_mm(Linv_r.to(dt), Ra))
+        apply_s(Ra, HRa)
+        G23 = gemm_nt(BUF[:m + ka], BUF[m:m + 2 * ka]).cpu()  # [Z; R_a] [R_a; HR_a]^T: (m + ka, 2ka)
+        # R_a HZ^T: (ka, m); equal to (Z HR_a^T)^T up to rounding when S is symmetric
+        G4 = None if symmetric else gemm_nt(Ra, HZ).cpu()
         mark("gram")
-        ZR, ZHR = G23[:m, :k], G23[:m, k:]
-        RR, RHR = G23[m:, :k], G23[m:, k:]
+        h0 = m + 2 * k - ka  # first row of [HR_a; HZ] for the S Z rotation
+        if not reapply and ka < k:
+            BUF[h0:m + 2 * k].copy_(HRa.clone() if m + 2 * ka > h0 else HRa)
+        ZR, ZHR = G23[:m, :ka], G23[:m, ka:]
+        RR, RHR = G23[m:, :ka], G23[m:, ka:]
         Pz = P.cpu() if P is not None else torch.diag(torch.from_numpy(theta_all))
         Bz = Gzz.cpu() if Gzz is not None else torch.eye(m, dtype=torch.float64)
         T = _sym(torch.cat([torch.cat([Pz, ZHR], 1), torch.cat([ZHR.T if G4 is None else G4, RHR], 1)], 0))
         GB = _sym(torch.cat([torch.cat([Bz, ZR], 1), torch.cat([ZR.T, RR], 1)], 0))
-        # generalized Rayleigh-Ritz on [Z; R]: c^T GB c = I, c^T T c = diag(theta)
+        # generalized Rayleigh-Ritz on [Z; R_a]: c^T GB c = I, c^T T c = diag(theta)
         thT, c = _host_geigh(T, GB)
-        if c.shape[1] < m + k:
+        if c.shape[1] < m + ka:
             res.truncated_steps += 1
         if c.shape[1] < m:
             raise RuntimeError("the Rayleigh-Ritz space lost rank below the working-subspace size")
         theta_all = thT[:m].numpy().copy()
-        Q = c[:, :m].T.contiguous().to(dev)  # new basis = Q [Z; R]
+        Q = c[:, :m].T.contiguous().to(dev)  # new basis = Q [Z; R_a]
         mark("eigh")
         Qd = Q.to(dt)
         out = BUF_next if BUF_next is not None else BUF
-        # Z' = Q [Z; R] and HZ' = Q [HR; HZ] (columns of Q reordered).  In place, each
+        # Z' = Q [Z; R_a] and HZ' = Q [HR_a; HZ] (columns of Q reordered).  In place, each
         # column panel of the sources is read whole before that panel is written, and the
-        # two rotations touch disjoint rows ([0, m + k) -> [0, m), [m + k, end) -> [m + 2k, end)).
-        rotate(Qd, BUF[:m + k], out[:m], BUF_next)
+        # two rotations touch disjoint rows ([0, m + ka) -> [0, m), [h0, end) -> [m + 2k, end)).
+        rotate(Qd, BUF[:m + ka], out[:m], BUF_next)
         if not reapply:
             Qh = torch.cat([Qd[:, m:], Qd[:, :m]], 1)
-            rotate(Qh, BUF[m + k:], out[m + 2 * k:], BUF_next)
+            rotate(Qh, BUF[h0:], out[m + 2 * k:], BUF_next)
         if BUF_next is not None:
             BUF, BUF_next = BUF_next, BUF
         if reapply:
             apply_s(BUF[:m], BUF[m + 2 * k:])
         res.eigenvalues, res.orbitals, res.residual_norms = th, Psi, rn
         res.iterations = it
+        res.two_buffers = res.two_buffers and BUF_next is not None
         res.ritz_matrix = ritz_k
     mark("rotation")
+    if profile:
+        t_it = res.iteration_seconds + [clock[0]]
+        res.iteration_seconds = [round(b - a, 5) for a, b in zip(t_it[:-1], t_it[1:])]
     res.timings = dict(tm)
     sw, isw = setup(dt)
     if keep_basis:

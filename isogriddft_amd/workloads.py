@@ -146,6 +146,8 @@ def run(name="benzene", n=224, k=None, block=None, tol=3e-4, multipole_order=4, 
         out[f"eig_{prec}_iterations"] = res.iterations
         out[f"eig_{prec}_converged"] = bool(res.converged)
         out[f"eig_{prec}_phase_s"] = {key: round(v, 3) for key, v in res.timings.items()}
+        # mean share of the k residual rows that entered the expansion (soft locking)
+        out[f"eig_{prec}_active_fraction"] = round(float(np.mean(res.active_rows)) / k, 3) if res.active_rows else 1.0
         ev[prec] = np.asarray(res.eigenvalues)
         if prec == "fp64":
             res64 = res

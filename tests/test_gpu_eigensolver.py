@@ -147,6 +147,44 @@ def test_small_working_subspace(precision):
     assert np.max(np.abs(res.eigenvalues - ref)) <= 1e-8
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("variant", ["two_buffers", "inplace", "reapply", "diagonal"])
+def test_soft_locking_matches_unlocked(precision, variant, monkeypatch):
+    """Soft locking (converged rows add no residual direction) reaches the dense
+    eigenvalues and those of the unlocked iteration, with the active S R rows moved in
+    front of S Z (two buffers and in-place panels), S Z re-applied, and the diagonal
+    preconditioner taking the active rows' Ritz values."""
+    g, V = _problem((14, 15, 16))
+    dg = DeviceGrid(g)
+    H = BlockHamiltonian(dg, torch.from_numpy(V).cuda(), closure="trial_fix")
+    k = 6
+    X0 = _start(dg, g, eigensolver.reference_block_size(k))
+    kw = {}
+    if variant == "inplace":
+        monkeypatch.setattr(eigensolver, "ROTATION_PANEL", 1000)
+        kw["inplace_rotation"] = True
+    elif variant == "reapply":
+        kw["hz_update"] = "apply"
+    elif variant == "diagonal":
+        kw["precondition"] = "diagonal"
+    monkeypatch.setattr(eigensolver, "LOCK_FRACTION", 0.5)  # lock early, so the tail runs with few rows
+    print("case", precision, variant, flush=True)
+    ref = eigensolver.block_subspace_iteration(H, dg.weights(), X0, k, 1e-7, 600, precision=precision, lock=False,
+                                               **kw)
+    got = eigensolver.block_subspace_iteration(H, dg.weights(), X0, k, 1e-7, 600, precision=precision, **kw)
+    dense = _dense_eigs(g, V, k)
+    print(precision, variant, ref.iterations, got.iterations, got.active_rows, got.eigenvalues - dense)
+    assert ref.converged and got.converged
+    assert min(got.active_rows) < k  # some rows were locked
+    assert max(ref.active_rows) == min(ref.active_rows) == k
+    assert np.max(np.abs(got.eigenvalues - dense)) <= 1e-8
+    assert np.max(np.abs(got.eigenvalues - ref.eigenvalues)) <= 1e-8
+    assert np.max(got.residual_norms) <= 1e-7
+    psi = got.orbitals.reshape(k, -1)
+    S = (psi * dg.weights()[None, :]) @ psi.T
+    assert torch.max(torch.abs(S - torch.eye(k, device=S.device, dtype=S.dtype))).item() <= 1e-12
+
+
 def test_orthonormalize_spans_oracle_subspace():
     g, _ = _problem()
     dg = DeviceGrid(g)
