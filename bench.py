@@ -289,11 +289,13 @@ MFMA_PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}
 def products_leg(n=224, m=192, k=48, reps=3):
     """MFMA utilisation of the eigensolver's two product shapes at the benzene block size
     (N = n^3, working subspace m, k states), timed with HIP events on the launch stream:
-    the basis rotation Z' = Q [Z; R] (2 m (m + k) N flop) and the Rayleigh-Ritz Gram block
-    [Z; R] [R; HR]^T (2 (m + k) 2k N flop), in FP32 and FP64, on seeded random blocks."""
+    the basis rotation Z' = Q [Z; R] (2 m (m + k) N flop) on the library's own MFMA kernel
+    (ig_rotate_*) and, for comparison, on the vendor GEMM (FP32 in 2^20-column panels), and
+    the Rayleigh-Ritz Gram block [Z; R] [R; HR]^T (2 (m + k) 2k N flop), in FP32 and FP64,
+    on seeded random blocks."""
     import torch
 
-    from isogriddft_amd.eigensolver import _gemm_nt_split, _mm
+    from isogriddft_amd.eigensolver import MM_PANEL, _gemm_nt_split, _rotate_kernel
 
     N = n ** 3
     out = {"shapes": f"N = {n}^3, rotation ({m} x {m + k}) x ({m + k} x N), Gram ({m + k} x N) x (N x {2 * k})"}
@@ -303,7 +305,17 @@ def products_leg(n=224, m=192, k=48, reps=3):
         B = torch.randn((m + k, N), dtype=dt, device="cuda", generator=g)
         Q = torch.randn((m, m + k), dtype=dt, device="cuda", generator=g)
         C = torch.empty((m, N), dtype=dt, device="cuda")
-        legs = (("rotation", lambda: _mm(Q, B, out=C), 2.0 * m * (m + k) * N),
+        def vendor_rotation():
+            for c0 in range(0, N, MM_PANEL if dt == torch.float32 else N):
+                c1 = min(c0 + MM_PANEL, N) if dt == torch.float32 else N
+                torch.matmul(Q, B[:, c0:c1], out=C[:, c0:c1])
+
+        def own_rotation():
+            if not _rotate_kernel(Q, B, C):
+                raise RuntimeError("the MFMA rotation kernel declined the benchmark shape")
+
+        legs = (("rotation", own_rotation, 2.0 * m * (m + k) * N),
+                ("rotation_vendor", vendor_rotation, 2.0 * m * (m + k) * N),
                 ("gram", lambda: _gemm_nt_split(B, B[:2 * k]), 2.0 * (m + k) * 2 * k * N))
         for what, fn, flop in legs:
             fn()

@@ -127,6 +127,18 @@ int ig_gram_f64(const double* X, long long ldx, int m, const double* Y, long lon
                 const double* w, long long N, double* G, int accumulate, double* work,
                 long long work_elems, hipStream_t stream);
 
+/* Y[M x N] = Q[M x K] X[K x N] on MFMA (FP32 v_mfma_f32_32x32x2f32, FP64
+ * v_mfma_f64_16x16x4f64): Q row-major (row stride ldq), X and Y row-major with row strides
+ * ldx / ldy.  Y may overlap X only row-aligned (ldy == ldx, offset by whole rows), e.g.
+ * Y = X for an in-place rotation of the first M rows.  M <= 256.  ct: column tiles per
+ * wavefront (0 automatic, 1, 2).  Replaces the basis rotations of the block iteration:
+ * ks/eigensolver_jax.py:184-201 (coefficients^T [basis; residuals]) and
+ * ks/eigensolver.py:916-932. */
+int ig_rotate_f32(const float* Q, int ldq, const float* X, long long ldx, float* Y, long long ldy, int M,
+                  int K, long long N, int ct, hipStream_t stream);
+int ig_rotate_f64(const double* Q, int ldq, const double* X, long long ldx, double* Y, long long ldy, int M,
+                  int K, long long N, int ct, hipStream_t stream);
+
 
 /* ------------------------------------------------------------------ physics fields */
 

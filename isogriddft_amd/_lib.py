@@ -38,6 +38,8 @@ SIGNATURES = {
     "ig_gram_work_size": [I, I, L],
     "ig_gram_f32": [P, L, I, P, L, I, P, L, P, I, P, L, P],
     "ig_gram_f64": [P, L, I, P, L, I, P, L, P, I, P, L, P],
+    "ig_rotate_f32": [P, I, P, L, P, L, I, I, L, I, P],
+    "ig_rotate_f64": [P, I, P, L, P, L, I, I, L, I, P],
     "ig_physics_work_size": [L],
     "ig_vloc_gth_f64": [P, L, I, P, P, I, P],
     "ig_vloc_gth_grad_f64": [P, P, L, I, P, P, P, P],

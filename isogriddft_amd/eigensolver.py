@@ -50,6 +50,7 @@ from . import configs
 from .device import DeviceGrid, device_grid
 from .hamiltonian import (BlockHamiltonian, H2_BENCHMARK_CASE, normalize_kinetic_version, normalize_spin_channel,
                           resolve_hartree, resolve_xc)
+from . import _lib
 from .kinetic import apply_operator
 from .reductions import cg_direction, cg_finish, cg_start, cg_update, gram, row_dots
 
@@ -63,15 +64,42 @@ FP32_STALL_CEILING = 100.0
 
 # widest column block handed to one FP32 library GEMM
 MM_PANEL = 1 << 20
+# rows of Q the MFMA rotation kernel accumulates in registers
+ROTATE_MAX_ROWS = 256
+
+
+def _rotate_kernel(A, B, out):
+    """out = A @ B on the library's MFMA rotation kernel (ig_rotate_*).  B and out are
+    row-major device blocks with unit column stride; out may overlap B row-aligned (the
+    same row stride, offset by whole rows), e.g. out = B[:M] for an in-place rotation.
+    Returns False (nothing launched) when the kernel does not take the shape (more than
+    ROTATE_MAX_ROWS rows of A, a non-unit column stride, a misaligned overlap)."""
+    M, K = A.shape
+    N = B.shape[1]
+    if (B.dtype not in (torch.float32, torch.float64) or out.dtype != B.dtype or M > ROTATE_MAX_ROWS
+            or B.stride(1) != 1 or out.stride(1) != 1 or out.shape != (M, N) or B.shape[0] != K):
+        return False
+    Ad = A.to(device=B.device, dtype=B.dtype).contiguous()
+    sfx = "f32" if B.dtype == torch.float32 else "f64"
+    st = getattr(_lib.load(), f"ig_rotate_{sfx}")(_lib.ptr(Ad), Ad.stride(0), _lib.ptr(B), B.stride(0), _lib.ptr(out),
+                                                   out.stride(0), M, K, N, 0, _lib.stream_handle())
+    if st == -3:
+        raise RuntimeError(f"ig_rotate_{sfx}: HIP launch failed")
+    return st == 0
 
 
 def _mm(A, B, out=None):
-    """A @ B for a small A (M, K) and a row-major block B (K, N).  FP32 products wider than
-    MM_PANEL columns run in column panels: the library's FP32 GEMM (hipBLASLt through
+    """A @ B for a small A (M, K) and a row-major block B (K, N).  Device blocks go to the
+    MFMA rotation kernel (up to ROTATE_MAX_ROWS rows of A).  Otherwise FP32 products wider
+    than MM_PANEL columns run in column panels: the library's FP32 GEMM (hipBLASLt through
     torch) was measured to return wrong values (NaN or ~1e17) for several (M, K), e.g.
     (192, 248) and (104, 152), at N = 224^3 in one call, and correct ones per panel of
     2^20 columns (FP64 and the rocBLAS path were correct at every shape tried)."""
     N = B.shape[1]
+    if B.is_cuda and A.shape[0] <= ROTATE_MAX_ROWS:
+        res = out if out is not None else torch.empty((A.shape[0], N), dtype=B.dtype, device=B.device)
+        if _rotate_kernel(A, B, res):
+            return res
     if B.dtype != torch.float32 or N <= MM_PANEL:
         return torch.matmul(A, B, out=out) if out is not None else torch.matmul(A, B)
     if out is None:
@@ -425,9 +453,14 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
         return torch.empty((rows, N), dtype=dtype, device=dev)
 
     def rotate(Qm, src, dst, spare):
-        """dst = Qm src, out of place into the spare buffer's rows or in column panels."""
+        """dst = Qm src, out of place into the spare buffer's rows, or in place: dst
+        overlaps src row-aligned, which the MFMA rotation kernel takes directly (each of
+        its wavefronts reads all of src in its columns before writing them); wider Q
+        falls back to column panels."""
         if spare is not None:
             _mm(Qm, src, out=dst)
+            return
+        if _rotate_kernel(Qm, src, dst):
             return
         for c0 in range(0, N, ROTATION_PANEL):
             c1 = min(c0 + ROTATION_PANEL, N)

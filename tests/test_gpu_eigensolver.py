@@ -93,7 +93,8 @@ def test_mixed_precision_refinement_matches_fp64(precondition):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_inplace_rotation_matches_two_buffers(monkeypatch, precision):
-    """The single-buffer form (column panels rotated in place) gives the two-buffer
+    """The single-buffer form (rotated in place by the MFMA kernel, or in column panels of
+    vendor products when the kernel does not take the shape) gives the two-buffer
     iteration's eigenvalues; a small panel width forces many panels and a ragged last one."""
     g, V = _problem((14, 15, 16))
     dg = DeviceGrid(g)
@@ -107,9 +108,14 @@ def test_inplace_rotation_matches_two_buffers(monkeypatch, precision):
     got = eigensolver.block_subspace_iteration(H, dg.weights(), X0, k, 1e-6, 400, precision=precision,
                                                inplace_rotation=True)
     print(precision, ref.iterations, got.iterations, got.eigenvalues - ref.eigenvalues)
-    assert ref.converged and got.converged
+    assert ref.converged and got.converged and not got.two_buffers
     assert np.max(np.abs(got.eigenvalues - ref.eigenvalues)) <= 1e-9
     assert got.orbitals.dtype == torch.float64
+    # the column-panel form (vendor products) that wider coefficient matrices take
+    monkeypatch.setattr(eigensolver, "ROTATE_MAX_ROWS", 0)
+    pan = eigensolver.block_subspace_iteration(H, dg.weights(), X0, k, 1e-6, 400, precision=precision,
+                                               inplace_rotation=True)
+    assert pan.converged and np.max(np.abs(pan.eigenvalues - ref.eigenvalues)) <= 1e-9
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
