@@ -621,7 +621,12 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
         mark("gram")
         h0 = m + 2 * k - ka  # first row of [HR_a; HZ] for the S Z rotation
         if not reapply and ka < k:
-            BUF[h0:m + 2 * k].copy_(HRa.clone() if m + 2 * ka > h0 else HRa)
+            # HR_a moves d = 2 (k - ka) rows down; overlapping ranges go back to front in
+            # blocks of at most d rows (each block's source is read before it is covered)
+            d = h0 - (m + ka)
+            for e in range(ka, 0, -d):
+                b0 = max(e - d, 0)
+                BUF[h0 + b0:h0 + e].copy_(BUF[m + ka + b0:m + ka + e])
         ZR, ZHR = G23[:m, :ka], G23[:m, ka:]
         RR, RHR = G23[m:, :ka], G23[m:, ka:]
         Pz = P.cpu() if P is not None else torch.diag(torch.from_numpy(theta_all))
