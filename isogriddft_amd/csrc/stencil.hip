@@ -1127,6 +1127,544 @@ int launch_monitor_rq_t(const MonParams<T>& base, hipStream_t stream, bool auto_
   return IG_OK;
 }
 
+// ---------------------------------------------------------------------------------
+// z-vectorised register-queue forward trial_fix stencil.  Every lane owns one vector
+// unit of V = 16 / sizeof(T) consecutive z points (128-bit global and LDS accesses) and
+// at most one scalar point of the tile's z halo.  A tile row is LZ lanes (TZ = 32
+// points); the 256 lanes are the PY = TY + 4 rows -2 .. TY + 1 of the plane (FP32: 8
+// lanes x 32 rows, TY = 28; FP64: 16 lanes x 16 rows, TY = 12), so every lane runs the
+// same instructions: interior rows produce output, rows -1 / TY only their y flux, rows
+// -2 / TY + 1 only plane values.  The scalar points: lanes lz = 0 / LZ - 1 of the interior
+// rows own the z-flux points k0 - 1 / k0 + TZ, of rows -1 / TY the plane values there;
+// lanes 1 / LZ - 2 of the interior rows the plane values at k0 - 2 / k0 + TZ + 1.  psi of
+// both points is kept for the planes s - 1 .. s + 2 in per-lane register queues (x
+// neighbours from registers), LDS holds the current plane (rows with their two z-halo
+// points on either side: every z and y neighbour is one LDS read) and the staged F_y /
+// F_z.  Same expression tree as the general kernel: bit-identical results.  Two barriers
+// per plane step (plane written; fluxes staged).
+template <typename T>
+struct QShape;
+template <>
+struct QShape<float> {
+  static constexpr int LZ = 8, TY = 28;
+};
+template <>
+struct QShape<double> {
+  static constexpr int LZ = 16, TY = 12;
+};
+
+template <typename T>
+__device__ __forceinline__ typename ZVec<T>::t lds_vec(const T* a, int i) {
+  return *reinterpret_cast<const typename ZVec<T>::t*>(a + i);
+}
+template <typename T>
+__device__ __forceinline__ void lds_vec_st(T* a, int i, typename ZVec<T>::t v) {
+  *reinterpret_cast<typename ZVec<T>::t*>(a + i) = v;
+}
+
+template <typename T, int NB>
+__global__ __launch_bounds__(IG_THREADS, 2) void monitor_zq_kernel(MonParams<T> p) {
+  using VT = typename ZVec<T>::t;
+  constexpr int V = ZVec<T>::V;
+  constexpr int LZ = QShape<T>::LZ, TY = QShape<T>::TY, TZ = LZ * V;
+  constexpr int PY = TY + 4;          // plane rows -2 .. TY + 1
+  constexpr int RP = TZ + 2 * V;      // plane row: [halo vector | TZ points | halo vector]
+  constexpr int PLANE = PY * RP;      // + one spare vector for lanes without a point
+  constexpr int FYR = TY + 2;         // F_y rows -1 .. TY, packed
+  constexpr int FYS = FYR * TZ;       // + spare vector
+  constexpr int FZS = TY * RP;        // F_z rows 0 .. TY - 1 with the k0 - 1 / k0 + TZ halo, + spare
+  constexpr unsigned S = sizeof(T);
+  static_assert(PY * LZ == IG_THREADS, "one plane vector per lane");
+  static_assert(TZ == 32, "32-point tile rows");
+
+  __shared__ __attribute__((aligned(16))) T s_pl[NB][PLANE + V];
+  __shared__ __attribute__((aligned(16))) T s_fy[NB][FYS + V];
+  __shared__ __attribute__((aligned(16))) T s_fz[NB][FZS + V];
+
+  const int ngroups = (p.nblk + NB - 1) / NB;
+  const int b = blockIdx.x;
+  const int xcd = b & 7;
+  const int r = b >> 3;
+  const int grp = r % ngroups;
+  const int upx = (p.units + 7) / 8;
+  const int u = xcd * upx + r / ngroups;
+  if (u >= p.units) return;
+  const int tzi = u % p.ntz;
+  const int tyi = (u / p.ntz) % p.nty;
+  const int xci = u / (p.ntz * p.nty);
+  const int nx = p.nx, ny = p.ny, nz = p.nz;
+  const unsigned plane_bytes = (unsigned)(ny * nz) * S;
+  const int j0 = max(min(tyi * TY, ny - TY), 0);
+  const int k0 = max(min(tzi * TZ, nz - TZ), 0);
+  const int x0 = xci * p.xc;
+  const int x1 = min(x0 + p.xc, nx);
+  const long long N = (long long)nx * ny * nz;
+
+  rsrc_t rin[NB], rout[NB];
+  bool orb_ok[NB];
+#pragma unroll
+  for (int n = 0; n < NB; ++n) {
+    const int orb = grp * NB + n;
+    orb_ok[n] = orb < p.nblk;
+    const int oc = orb_ok[n] ? orb : p.nblk - 1;
+    rin[n] = mkbuf(p.in + (long long)oc * p.ld, N);
+    rout[n] = mkbuf(p.out + (long long)oc * p.ld, N);
+  }
+  const rsrc_t rxx = mkbuf(p.coef, N), ryy = mkbuf(p.coef + N, N), rzz = mkbuf(p.coef + 2 * N, N);
+  const rsrc_t rxy = mkbuf(p.coef + 3 * N, N), rxz = mkbuf(p.coef + 4 * N, N), ryz = mkbuf(p.coef + 5 * N, N);
+  const rsrc_t rij = mkbuf(p.invj, N);
+  const rsrc_t rv = mkbuf(p.veff ? p.veff : p.invj, p.veff ? N : 0);
+
+  const int tid = threadIdx.x;
+  // ---- vector unit: row r of the plane (-2 .. TY + 1), z points k0 + V lz .. + V - 1
+  const int row = tid / LZ - 2, lz = tid % LZ;
+  const int gj = j0 + row;
+  const bool jok = gj >= 0 && gj < ny;
+  const bool kok = k0 + V * lz < nz;  // nz % V == 0: all V points in or all out
+  const unsigned voff = (jok && kok) ? (unsigned)(gj * nz + k0 + V * lz) * S : BAD;
+  const bool interior = row >= 0 && row < TY;
+  const bool yflux = row >= -1 && row <= TY;
+  const unsigned goff = yflux ? voff : BAD;      // flux coefficients (rows -1 .. TY)
+  const unsigned ooff = interior ? voff : BAD;   // 1/J, V and the output
+  const int pw = (row + 2) * RP + V + V * lz;    // plane position of the vector
+  const int rm = max(row - 1, -2), rp = min(row + 1, TY + 1);
+  const int pym = (rm + 2) * RP + V + V * lz, pyp = (rp + 2) * RP + V + V * lz;
+  const int fyw = yflux ? (row + 1) * TZ + V * lz : FYS;
+  const int cr = min(max(row, 0), TY - 1);       // clamped row for the divergence reads
+  const int fym = cr * TZ + V * lz, fyp = (cr + 2) * TZ + V * lz;
+  const int fzw = interior ? row * RP + V + V * lz : FZS;
+  const int fzc = cr * RP + V + V * lz;
+  // ---- scalar point of the z halo
+  int ecol = 0, epos = PLANE, efz = FZS;
+  bool eflux = false, ehas = false;
+  if (interior || row == -1 || row == TY) {
+    if (lz == 0 || lz == LZ - 1) {
+      const bool left = lz == 0;
+      ecol = left ? k0 - 1 : k0 + TZ;
+      epos = (row + 2) * RP + (left ? V - 1 : V + TZ);
+      ehas = true;
+      if (interior) {
+        eflux = true;
+        efz = row * RP + (left ? V - 1 : V + TZ);
+      }
+    } else if (interior && (lz == 1 || lz == LZ - 2)) {
+      const bool left = lz == 1;
+      ecol = left ? k0 - 2 : k0 + TZ + 1;
+      epos = (row + 2) * RP + (left ? V - 2 : V + TZ + 1);
+      ehas = true;
+    }
+  }
+  const unsigned eoff = (ehas && jok && ecol >= 0 && ecol < nz) ? (unsigned)(gj * nz + ecol) * S : BAD;
+  const unsigned hoff = eflux ? eoff : BAD;
+  const int ec = eflux ? epos : RP + 1;  // neighbour reads of non-flux lanes stay inside the plane
+
+  auto pbytes = [&](int q) { return (q >= 0 && q < nx) ? (unsigned)q * plane_bytes : BAD; };
+  const T tag = T(0);
+
+  VT q[NB][4];    // psi of the vector unit, slot of plane q: (q - sb + 1) & 3
+  T qe[NB][4];    // psi of the scalar point, same slots
+  VT fx[NB][2];   // F_x, slot (q - sb) & 1: F_x(s) replaces F_x(s - 2)
+  VT dyz[NB];     // y/z divergence of the previous plane
+  VT gc[6];       // flux coefficients of the current plane
+  T hc[3];        // z-flux coefficients of the scalar point
+  VT oij, ov;     // 1/J and V of the output plane
+  auto fetch_psi = [&](int qq, int slot) {
+    const unsigned pb = pbytes(qq);
+#pragma unroll
+    for (int n = 0; n < NB; ++n) {
+      q[n][slot] = zld<T>(rin[n], pb + voff);
+      qe[n][slot] = bld(rin[n], pb + eoff, tag);
+    }
+  };
+  auto fetch_geom = [&](int qq) {
+    const unsigned o = pbytes(qq) + goff;
+    gc[0] = zld<T>(rxx, o); gc[1] = zld<T>(ryy, o); gc[2] = zld<T>(rzz, o);
+    gc[3] = zld<T>(rxy, o); gc[4] = zld<T>(rxz, o); gc[5] = zld<T>(ryz, o);
+    const unsigned h = pbytes(qq) + hoff;
+    hc[0] = bld(rxz, h, tag); hc[1] = bld(ryz, h, tag); hc[2] = bld(rzz, h, tag);
+  };
+  auto fetch_out = [&](int qq) {
+    const unsigned o = pbytes(qq) + ooff;
+    oij = zld<T>(rij, o);
+    ov = zld<T>(rv, o);
+  };
+
+  const int sb = x0 - 1;  // steps sb .. x1, output plane o = s - 1 in [x0, x1)
+#pragma unroll
+  for (int n = 0; n < NB; ++n) {
+    fx[n][0] = fx[n][1] = VT(0);
+    dyz[n] = VT(0);
+  }
+#pragma unroll
+  for (int d = 0; d < 4; ++d) fetch_psi(sb - 1 + d, d);
+  fetch_geom(sb);
+  fetch_out(sb - 1);
+  const T sc = p.scale;
+
+  // Whole groups of 4 steps, no exit inside the unrolled body (a branch there makes the
+  // compiler drain every outstanding load at the join); steps past x1 load zeros and
+  // store nothing.
+  for (int s0 = sb; s0 <= x1; s0 += 4) {
+#pragma unroll
+    for (int U = 0; U < 4; ++U) {
+      const int s = s0 + U;
+      const int sm = U & 3, s0l = (U + 1) & 3, sp = (U + 2) & 3;  // slots of s-1, s, s+1
+      const int W = U & 1;
+      // ---- phase A: publish plane s
+#pragma unroll
+      for (int n = 0; n < NB; ++n) {
+        lds_vec_st<T>(s_pl[n], pw, q[n][s0l]);
+        s_pl[n][epos] = qe[n][s0l];
+      }
+      __syncthreads();
+      // ---- phase B: gradients and fluxes of plane s
+      VT fxs[NB], fzs[NB];
+#pragma unroll
+      for (int n = 0; n < NB; ++n) {
+        const T* pl = s_pl[n];
+        const VT c = q[n][s0l];
+        const VT gx = q[n][sp] - q[n][sm];
+        const VT gy = lds_vec<T>(pl, pyp) - lds_vec<T>(pl, pym);
+        const T zl = pl[pw - 1], zr = pl[pw + V];
+        VT gz, f2;
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          gz[e] = (e + 1 < V ? c[e + 1] : zr) - (e > 0 ? c[e - 1] : zl);
+          fxs[n][e] = flux3(gc[0][e], gx[e], gc[3][e], gy[e], gc[4][e], gz[e]);
+          f2[e] = flux3(gc[3][e], gx[e], gc[1][e], gy[e], gc[5][e], gz[e]);
+          fzs[n][e] = flux3(gc[4][e], gx[e], gc[5][e], gy[e], gc[2][e], gz[e]);
+        }
+        lds_vec_st<T>(s_fy[n], fyw, f2);
+        lds_vec_st<T>(s_fz[n], fzw, fzs[n]);
+        // keep F_x here: evaluated after the barrier it would hold the three gradients live
+        asm volatile("" : "+v"(fxs[n]));
+        {
+          const T egx = qe[n][sp] - qe[n][sm];
+          const T egy = pl[ec + RP] - pl[ec - RP];
+          const T egz = pl[ec + 1] - pl[ec - 1];
+          s_fz[n][efz] = flux3(hc[0], egx, hc[1], egy, hc[2], egz);
+        }
+      }
+      fetch_geom(s + 1);
+      __syncthreads();
+      // ---- phase C: y/z divergence of plane s, output plane o = s - 1
+      const int o = s - 1;
+      const unsigned ob = (o >= x0 && o < x1) ? (unsigned)o * plane_bytes : BAD;
+#pragma unroll
+      for (int n = 0; n < NB; ++n) {
+        const VT fyp_v = lds_vec<T>(s_fy[n], fyp), fym_v = lds_vec<T>(s_fy[n], fym);
+        const T fzl = s_fz[n][fzc - 1], fzr = s_fz[n][fzc + V];
+        const VT fz = fzs[n];
+        const VT dx = fxs[n] - fx[n][W];
+        fx[n][W] = fxs[n];
+        const VT po = q[n][sm];
+        VT d, v;
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          d[e] = (fyp_v[e] - fym_v[e]) + ((e + 1 < V ? fz[e + 1] : fzr) - (e > 0 ? fz[e - 1] : fzl));
+          const T lap = dx[e] + dyz[n][e];
+          v[e] = fma(ov[e], po[e], sc * (lap * oij[e]));
+        }
+        dyz[n] = d;
+        zst<T>(rout[n], (orb_ok[n] ? ob : BAD) + ooff, v);
+      }
+      fetch_out(s);
+      fetch_psi(s + 3, sm);
+    }
+  }
+}
+
+template <typename T, int NB>
+int launch_monitor_zq_t(const MonParams<T>& base, hipStream_t stream, bool auto_pick = false) {
+  constexpr int V = ZVec<T>::V;
+  constexpr int TY = QShape<T>::TY, TZ = QShape<T>::LZ * V;
+  MonParams<T> p = base;
+  if ((long long)p.nx * p.ny * p.nz * (long long)sizeof(T) >= (1LL << 30)) return IG_EUNSUPPORTED;
+  if (p.interior_only) return IG_EUNSUPPORTED;
+  // 128-bit accesses: every z row and every orbital must start on a 16-byte boundary
+  if (p.nz % V != 0 || p.ld % V != 0) return IG_EUNSUPPORTED;
+  if (((uintptr_t)p.in | (uintptr_t)p.out | (uintptr_t)p.coef | (uintptr_t)p.invj | (uintptr_t)p.veff) & 15)
+    return IG_EUNSUPPORTED;
+  p.nty = (p.ny + TY - 1) / TY;
+  p.ntz = (p.nz + TZ - 1) / TZ;
+  const long long groups = (p.nblk + NB - 1) / NB;
+  if (p.xc <= 0) {
+    // chunks of ~56 planes, rounded to 4 m + 2 planes: a chunk runs its planes plus two
+    // halo steps in whole groups of 4 (224 planes: 58, 58, 58, 50)
+    const int c = p.nx >= 112 ? (p.nx + 28) / 56 : 1;
+    p.xc = (p.nx + c - 1) / c;
+    p.xc = ((p.xc + 1) / 4) * 4 + 2;
+  }
+  if (p.xc > p.nx) p.xc = p.nx;
+  p.nxc = (p.nx + p.xc - 1) / p.xc;
+  p.units = p.nty * p.ntz * p.nxc;
+  const long long blocks = 8LL * groups * ((p.units + 7) / 8);
+  if (blocks <= 0 || blocks > 0x7fffffffLL) return IG_EINVAL;
+  if (auto_pick && blocks < 2048) return IG_EUNSUPPORTED;
+  hipLaunchKernelGGL((monitor_zq_kernel<T, NB>), dim3((unsigned)blocks), dim3(IG_THREADS), 0, stream, p);
+  IG_CHECK_LAUNCH();
+  return IG_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// Pipelined full-z register-queue forward trial_fix stencil, for grids whose z rows fit
+// one wavefront (nz <= 64 V: FP32 256, FP64 128 points; the default for large FP32 blocks).
+// A tile spans the whole z extent, so no z halo is ever fetched.  (With 32-point tiles
+// every tile row is one 128-byte line and its four z-halo points pull the two
+// neighbouring lines from HBM: at 224^3, k = 48 that doubled the psi read traffic, and
+// the z-neighbour tile's workgroup does not reliably hit them in L2.)  Wavefront w owns
+// plane row w - 2 of a TY-row tile: rows 0 .. TY - 1 produce output, rows -1 / TY only
+// their y flux, rows -2 / TY + 1 only plane values (wavefront-uniform branches); lane l
+// owns the V points V l .. V l + V - 1.  psi of the lane's points for the planes
+// s - 1 .. s + 2 is kept in register queues (x neighbours), z neighbours of psi and of
+// F_z come from the adjacent lane by lane permutes (ds_bpermute: no LDS storage, no bank
+// conflicts; the zero ghosts at k = -1 / nz are the permute of lane 0 / 63 masked, or a
+// lane past nz holding zeros), and LDS holds only the plane and F_y, both double
+// buffered: step s computes the fluxes of plane s and the divergence and output of plane
+// s - 1, publishes plane s + 1, and ends with the step's only barrier.  The work per
+// step that does not depend on the orbital (6 + 2 coefficient loads per lane, the
+// barrier) is shared by the NB orbitals of the workgroup, so NB = 3 is used: 12
+// wavefronts at <= 168 VGPRs (3 per SIMD) and 144 KB of LDS fill a CU.  Same expression
+// tree as the general kernel: bit-identical results.
+template <typename T>
+__device__ __forceinline__ T lane_from(int addr, T v);
+template <>
+__device__ __forceinline__ float lane_from<float>(int addr, float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(addr, __builtin_bit_cast(int, v)));
+}
+template <>
+__device__ __forceinline__ double lane_from<double>(int addr, double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)(unsigned)(b & 0xffffffffull));
+  const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(unsigned)(b >> 32));
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo);
+}
+
+template <typename T, int NB, int GD, int TY>
+__global__ __launch_bounds__((TY + 4) * 64) void monitor_zp_kernel(MonParams<T> p) {
+  using VT = typename ZVec<T>::t;
+  constexpr int V = ZVec<T>::V;
+  constexpr int PY = TY + 4;   // plane rows -2 .. TY + 1, one per wavefront
+  constexpr int NT = PY * 64;
+  constexpr int RW = 64 * V;   // points per row (lanes past nz hold zeros)
+  constexpr unsigned S = sizeof(T);
+  static_assert(NT <= 1024, "one wavefront per plane row");
+  static_assert(GD == 1 || GD == 2, "geometry one or two planes ahead");
+
+  __shared__ __attribute__((aligned(16))) T s_pl[2][NB][PY * RW];
+  __shared__ __attribute__((aligned(16))) T s_fy[2][NB][PY * RW];
+
+  const int ngroups = (p.nblk + NB - 1) / NB;
+  const int b = blockIdx.x;
+  const int xcd = b & 7;
+  const int r = b >> 3;
+  const int grp = r % ngroups;
+  const int upx = (p.units + 7) / 8;
+  const int u = xcd * upx + r / ngroups;
+  if (u >= p.units) return;
+  const int tyi = u % p.nty;
+  const int xci = u / p.nty;
+  const int nx = p.nx, ny = p.ny, nz = p.nz;
+  const unsigned plane_bytes = (unsigned)(ny * nz) * S;
+  const int j0 = max(min(tyi * TY, ny - TY), 0);
+  const int x0 = xci * p.xc;
+  const int x1 = min(x0 + p.xc, nx);
+  const long long N = (long long)nx * ny * nz;
+
+  rsrc_t rin[NB], rout[NB];
+  bool orb_ok[NB];
+#pragma unroll
+  for (int n = 0; n < NB; ++n) {
+    const int orb = grp * NB + n;
+    orb_ok[n] = orb < p.nblk;
+    const int oc = orb_ok[n] ? orb : p.nblk - 1;
+    rin[n] = mkbuf(p.in + (long long)oc * p.ld, N);
+    rout[n] = mkbuf(p.out + (long long)oc * p.ld, N);
+  }
+  const rsrc_t rxx = mkbuf(p.coef, N), ryy = mkbuf(p.coef + N, N), rzz = mkbuf(p.coef + 2 * N, N);
+  const rsrc_t rxy = mkbuf(p.coef + 3 * N, N), rxz = mkbuf(p.coef + 4 * N, N), ryz = mkbuf(p.coef + 5 * N, N);
+  const rsrc_t rij = mkbuf(p.invj, N);
+  const rsrc_t rv = mkbuf(p.veff ? p.veff : p.invj, p.veff ? N : 0);
+
+  const int tid = threadIdx.x;
+  const int row = __builtin_amdgcn_readfirstlane(tid >> 6) - 2;  // uniform per wavefront
+  const int ln = tid & 63;
+  const int gj = j0 + row;
+  const bool jok = gj >= 0 && gj < ny;
+  const bool kok = V * ln < nz;  // nz % V == 0: all V points in or all out
+  const unsigned voff = (jok && kok) ? (unsigned)(gj * nz + V * ln) * S : BAD;
+  const bool interior = row >= 0 && row < TY;
+  const bool yflux = row >= -1 && row <= TY;
+  const unsigned goff = yflux ? voff : BAD;
+  const unsigned ooff = interior ? voff : BAD;
+  const int pw = (row + 2) * RW + V * ln;  // LDS position of the lane's vector (plane and F_y)
+  // y neighbours; the outermost wavefronts (whose results are unused) read their own row
+  const int pym = row > -2 ? pw - RW : pw, pyp = row < TY + 1 ? pw + RW : pw;
+  const int al = ((ln + 63) & 63) * 4, ar = ((ln + 1) & 63) * 4;  // permute addresses of the z neighbours
+  const bool zl_ghost = ln == 0, zr_ghost = ln == 63;  // lanes past nz hold zeros already
+
+  auto pbytes = [&](int q) { return (q >= 0 && q < nx) ? (unsigned)q * plane_bytes : BAD; };
+
+  VT q[NB][4];    // psi, slot of plane q: (q - sb + 1) & 3
+  VT fx[NB][2];   // F_x, slot (q - sb) & 1
+  VT dz[NB];      // z part of the divergence of the previous plane
+  VT gc[GD][6];   // flux coefficients, set (q - sb) % GD
+  VT oij, ov;     // 1/J and V of the output plane
+  auto fetch_psi = [&](int qq, int slot) {
+    const unsigned pb = pbytes(qq);
+#pragma unroll
+    for (int n = 0; n < NB; ++n) q[n][slot] = zld<T>(rin[n], pb + voff);
+  };
+  // wavefront-uniform roles: the y-halo rows need only the F_y coefficients, the outer rows
+  // no geometry at all, and only interior rows read 1/J and V
+  auto fetch_geom = [&](int qq, int w) {
+    const unsigned o = pbytes(qq) + goff;
+    if (interior) {
+      gc[w][0] = zld<T>(rxx, o); gc[w][2] = zld<T>(rzz, o); gc[w][4] = zld<T>(rxz, o);
+    }
+    if (yflux) {
+      gc[w][1] = zld<T>(ryy, o); gc[w][3] = zld<T>(rxy, o); gc[w][5] = zld<T>(ryz, o);
+    }
+  };
+  auto fetch_out = [&](int qq) {
+    if (interior) {
+      const unsigned o = pbytes(qq) + ooff;
+      oij = zld<T>(rij, o);
+      ov = zld<T>(rv, o);
+    }
+  };
+
+  const int sb = x0 - 1;  // steps sb .. x1 (whole groups of 4), output plane o = s - 1 in [x0, x1)
+#pragma unroll
+  for (int n = 0; n < NB; ++n) {
+    fx[n][0] = fx[n][1] = VT(0);
+    dz[n] = VT(0);
+  }
+#pragma unroll
+  for (int d = 0; d < 4; ++d) fetch_psi(sb - 1 + d, d);
+#pragma unroll
+  for (int d = 0; d < GD; ++d) fetch_geom(sb + d, d);
+  fetch_out(sb - 1);
+  const T sc = p.scale;
+  // plane sb into buffer 0 (slot 1); buffer of plane q: (q - sb) & 1
+#pragma unroll
+  for (int n = 0; n < NB; ++n) lds_vec_st<T>(s_pl[0][n], pw, q[n][1]);
+  __syncthreads();
+
+  for (int s0 = sb; s0 <= x1; s0 += 4) {
+#pragma unroll
+    for (int U = 0; U < 4; ++U) {
+      const int s = s0 + U;
+      const int sm = U & 3, s0l = (U + 1) & 3, sp = (U + 2) & 3;  // slots of s-1, s, s+1
+      const int W = U & 1;                                         // buffers / F_x slot of plane s
+      const int G = GD == 2 ? W : 0;
+      const int o = s - 1;
+      const unsigned ob = (o >= x0 && o < x1) ? (unsigned)o * plane_bytes : BAD;
+      if (interior) {
+#pragma unroll
+      for (int n = 0; n < NB; ++n) {
+        // ---- fluxes of plane s
+        const T* pl = s_pl[W][n];
+        const VT c = q[n][s0l];
+        const VT gx = q[n][sp] - q[n][sm];
+        const VT gy = lds_vec<T>(pl, pyp) - lds_vec<T>(pl, pym);
+        T zl = lane_from<T>(al, c[V - 1]), zr = lane_from<T>(ar, c[0]);
+        zl = zl_ghost ? T(0) : zl;
+        zr = zr_ghost ? T(0) : zr;
+        VT gz, fxs, f2, f3;
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          gz[e] = (e + 1 < V ? c[e + 1] : zr) - (e > 0 ? c[e - 1] : zl);
+          fxs[e] = flux3(gc[G][0][e], gx[e], gc[G][3][e], gy[e], gc[G][4][e], gz[e]);
+          f2[e] = flux3(gc[G][3][e], gx[e], gc[G][1][e], gy[e], gc[G][5][e], gz[e]);
+          f3[e] = flux3(gc[G][4][e], gx[e], gc[G][5][e], gy[e], gc[G][2][e], gz[e]);
+        }
+        lds_vec_st<T>(s_fy[W][n], pw, f2);
+        T fzl = lane_from<T>(al, f3[V - 1]), fzr = lane_from<T>(ar, f3[0]);
+        fzl = zl_ghost ? T(0) : fzl;
+        fzr = zr_ghost ? T(0) : fzr;
+        VT dzn;
+#pragma unroll
+        for (int e = 0; e < V; ++e) dzn[e] = (e + 1 < V ? f3[e + 1] : fzr) - (e > 0 ? f3[e - 1] : fzl);
+        // ---- divergence and output of plane s - 1 (F_y staged in the previous step)
+        const T* fyo = s_fy[W ^ 1][n];
+        const VT fyp_v = lds_vec<T>(fyo, pyp), fym_v = lds_vec<T>(fyo, pym);
+        const VT dx = fxs - fx[n][W];
+        fx[n][W] = fxs;
+        const VT po = q[n][sm];
+        VT v;
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          const T d = (fyp_v[e] - fym_v[e]) + dz[n][e];
+          const T lap = dx[e] + d;
+          v[e] = fma(ov[e], po[e], sc * (lap * oij[e]));
+        }
+        dz[n] = dzn;
+        zst<T>(rout[n], (orb_ok[n] ? ob : BAD) + ooff, v);
+        // ---- publish plane s + 1
+        lds_vec_st<T>(s_pl[W ^ 1][n], pw, q[n][sp]);
+      }
+      } else if (yflux) {
+        // y-halo rows: F_y of plane s only
+#pragma unroll
+        for (int n = 0; n < NB; ++n) {
+          const T* pl = s_pl[W][n];
+          const VT c = q[n][s0l];
+          const VT gx = q[n][sp] - q[n][sm];
+          const VT gy = lds_vec<T>(pl, pyp) - lds_vec<T>(pl, pym);
+          T zl = lane_from<T>(al, c[V - 1]), zr = lane_from<T>(ar, c[0]);
+          zl = zl_ghost ? T(0) : zl;
+          zr = zr_ghost ? T(0) : zr;
+          VT f2;
+#pragma unroll
+          for (int e = 0; e < V; ++e) {
+            const T gz = (e + 1 < V ? c[e + 1] : zr) - (e > 0 ? c[e - 1] : zl);
+            f2[e] = flux3(gc[G][3][e], gx[e], gc[G][1][e], gy[e], gc[G][5][e], gz);
+          }
+          lds_vec_st<T>(s_fy[W][n], pw, f2);
+          lds_vec_st<T>(s_pl[W ^ 1][n], pw, q[n][sp]);
+        }
+      } else {
+#pragma unroll
+        for (int n = 0; n < NB; ++n) lds_vec_st<T>(s_pl[W ^ 1][n], pw, q[n][sp]);
+      }
+      fetch_geom(s + GD, G);
+      fetch_out(s);
+      fetch_psi(s + 3, sm);
+      __syncthreads();
+    }
+  }
+}
+
+template <typename T, int NB, int GD, int TY>
+int launch_monitor_zp_t(const MonParams<T>& base, hipStream_t stream, bool auto_pick = false) {
+  constexpr int V = ZVec<T>::V;
+  MonParams<T> p = base;
+  if ((long long)p.nx * p.ny * p.nz * (long long)sizeof(T) >= (1LL << 30)) return IG_EUNSUPPORTED;
+  if (p.interior_only) return IG_EUNSUPPORTED;
+  if (p.nz > 64 * V || p.nz % V != 0 || p.ld % V != 0) return IG_EUNSUPPORTED;
+  if (((uintptr_t)p.in | (uintptr_t)p.out | (uintptr_t)p.coef | (uintptr_t)p.invj | (uintptr_t)p.veff) & 15)
+    return IG_EUNSUPPORTED;
+  p.nty = (p.ny + TY - 1) / TY;
+  p.ntz = 1;
+  const long long groups = (p.nblk + NB - 1) / NB;
+  if (p.xc <= 0) {
+    const int c = p.nx >= 112 ? (p.nx + 28) / 56 : 1;
+    p.xc = (p.nx + c - 1) / c;
+    p.xc = ((p.xc + 1) / 4) * 4 + 2;
+  }
+  if (p.xc > p.nx) p.xc = p.nx;
+  p.nxc = (p.nx + p.xc - 1) / p.xc;
+  p.units = p.nty * p.nxc;
+  const long long blocks = 8LL * groups * ((p.units + 7) / 8);
+  if (blocks <= 0 || blocks > 0x7fffffffLL) return IG_EINVAL;
+  if (auto_pick && blocks < 512) return IG_EUNSUPPORTED;
+  hipLaunchKernelGGL((monitor_zp_kernel<T, NB, GD, TY>), dim3((unsigned)blocks), dim3((TY + 4) * 64), 0, stream, p);
+  IG_CHECK_LAUNCH();
+  return IG_OK;
+}
+
+
 
 #define IG_MON_CASE(TY_, TZ_, NB_, PD_)                                                            \
   if (ty == TY_ && tz == TZ_ && nb == NB_ && pd == PD_)                                            \
@@ -1140,6 +1678,29 @@ template <typename T>
 int launch_monitor(const MonParams<T>& p, int closure, int transpose, int tile, hipStream_t s) {
   if (transpose) return closure ? launch_monitor_t<T, 8, 32, 1, true>(p, s) : launch_monitor_t<T, 8, 32, 0, true>(p, s);
   if (tile < 0) return IG_EINVAL;
+  if (tile >= 9200000) {
+    if (closure) return IG_EUNSUPPORTED;
+    switch (tile - 9200000) {
+      // rows per tile x 10 + orbitals per workgroup (+ 100: coefficients two planes ahead)
+      case 81: return launch_monitor_zp_t<T, 1, 1, 8>(p, s);
+      case 82: return launch_monitor_zp_t<T, 2, 1, 8>(p, s);
+      case 83: return launch_monitor_zp_t<T, 3, 1, 8>(p, s);
+      case 182: return launch_monitor_zp_t<T, 2, 2, 8>(p, s);
+      case 122: return launch_monitor_zp_t<T, 2, 1, 12>(p, s);
+      case 43: return launch_monitor_zp_t<T, 3, 1, 4>(p, s);
+      default: return IG_EUNSUPPORTED;
+    }
+  }
+  if (tile >= 9000000) {
+    if (closure) return IG_EUNSUPPORTED;
+    switch (tile - 9000000) {
+      case 1: return launch_monitor_zq_t<T, 1>(p, s);
+      case 2: return launch_monitor_zq_t<T, 2>(p, s);
+      case 3: return launch_monitor_zq_t<T, 3>(p, s);
+      case 4: return launch_monitor_zq_t<T, 4>(p, s);
+      default: return IG_EUNSUPPORTED;
+    }
+  }
   if (tile >= 8000000) {
     if (closure) return IG_EUNSUPPORTED;
     switch (tile - 8000000) {
@@ -1160,11 +1721,17 @@ int launch_monitor(const MonParams<T>& p, int closure, int transpose, int tile, 
   // orbitals with one geometry fetch and still fit 3 / 2 to a CU (224^3, k = 48: FP32
   // 1.69 ms against 1.90 ms for the general kernel, FP64 3.22 ms against 3.70 ms for the
   // z-vectorised one).
+  // Large blocks (geometry beyond the Infinity Cache): FP32 rows that fit one wavefront
+  // take the pipelined full-z kernel (224^3, k = 48: 1.24 ms against 1.67 ms for the
+  // register-queue kernel), FP64 the z-vectorised register-queue kernel (3.03 ms against
+  // 3.22 ms); shapes these decline fall back to the register-queue kernel.
   if (tile == 0 && closure == 0 && !transpose && !p.interior_only &&
       (long long)p.nx * p.ny * p.nz * (long long)sizeof(T) * 8 > (128LL << 20)) {
     int st = IG_EUNSUPPORTED;
-    if (sizeof(T) == 4 && p.nblk >= 6) st = launch_monitor_rq_t<T, 6>(p, s, true);
-    else if (sizeof(T) == 8 && p.nblk >= 4) st = launch_monitor_rq_t<T, 4>(p, s, true);
+    if (sizeof(T) == 4 && p.nblk >= 3 && p.nz >= 128) st = launch_monitor_zp_t<T, 3, 1, 8>(p, s, true);
+    if (st == IG_EUNSUPPORTED && sizeof(T) == 8 && p.nblk >= 3) st = launch_monitor_zq_t<T, 3>(p, s, true);
+    if (st == IG_EUNSUPPORTED && sizeof(T) == 4 && p.nblk >= 6) st = launch_monitor_rq_t<T, 6>(p, s, true);
+    else if (st == IG_EUNSUPPORTED && sizeof(T) == 8 && p.nblk >= 4) st = launch_monitor_rq_t<T, 4>(p, s, true);
     if (st != IG_EUNSUPPORTED) return st;
   }
   if (tile == 0 && closure == 0 && !fp32_big) {
