@@ -1505,7 +1505,11 @@ __global__ __launch_bounds__((TY + 4) * 64) void monitor_zp_kernel(MonParams<T> 
 
   auto pbytes = [&](int q) { return (q >= 0 && q < nx) ? (unsigned)q * plane_bytes : BAD; };
 
-  VT q[NB][4];    // psi, slot of plane q: (q - sb + 1) & 3
+  // psi in registers only for the plane being published and the one in flight, slot of
+  // plane q: (q - sb) & 1; a lane reads its own psi of planes s - 1 and s back from its
+  // slots of the two plane buffers (plane s - 1's slot is overwritten by the lane itself
+  // when it publishes plane s + 1, after that read)
+  VT q[NB][2];
   VT fx[NB][2];   // F_x, slot (q - sb) & 1
   VT dz[NB];      // z part of the divergence of the previous plane
   VT gc[GD][6];   // flux coefficients, set (q - sb) % GD
@@ -1540,23 +1544,27 @@ __global__ __launch_bounds__((TY + 4) * 64) void monitor_zp_kernel(MonParams<T> 
     fx[n][0] = fx[n][1] = VT(0);
     dz[n] = VT(0);
   }
-#pragma unroll
-  for (int d = 0; d < 4; ++d) fetch_psi(sb - 1 + d, d);
+  fetch_psi(sb - 1, 1);
+  fetch_psi(sb, 0);
 #pragma unroll
   for (int d = 0; d < GD; ++d) fetch_geom(sb + d, d);
   fetch_out(sb - 1);
   const T sc = p.scale;
-  // plane sb into buffer 0 (slot 1); buffer of plane q: (q - sb) & 1
+  // planes sb - 1 and sb into their buffers; buffer of plane q: (q - sb) & 1
 #pragma unroll
-  for (int n = 0; n < NB; ++n) lds_vec_st<T>(s_pl[0][n], pw, q[n][1]);
+  for (int n = 0; n < NB; ++n) {
+    lds_vec_st<T>(s_pl[1][n], pw, q[n][1]);
+    lds_vec_st<T>(s_pl[0][n], pw, q[n][0]);
+  }
+  fetch_psi(sb + 1, 1);
+  fetch_psi(sb + 2, 0);
   __syncthreads();
 
   for (int s0 = sb; s0 <= x1; s0 += 4) {
 #pragma unroll
     for (int U = 0; U < 4; ++U) {
       const int s = s0 + U;
-      const int sm = U & 3, s0l = (U + 1) & 3, sp = (U + 2) & 3;  // slots of s-1, s, s+1
-      const int W = U & 1;                                         // buffers / F_x slot of plane s
+      const int W = U & 1;  // buffer / F_x slot of plane s; psi(s + 1) in register slot W ^ 1
       const int G = GD == 2 ? W : 0;
       const int o = s - 1;
       const unsigned ob = (o >= x0 && o < x1) ? (unsigned)o * plane_bytes : BAD;
@@ -1565,8 +1573,9 @@ __global__ __launch_bounds__((TY + 4) * 64) void monitor_zp_kernel(MonParams<T> 
       for (int n = 0; n < NB; ++n) {
         // ---- fluxes of plane s
         const T* pl = s_pl[W][n];
-        const VT c = q[n][s0l];
-        const VT gx = q[n][sp] - q[n][sm];
+        const VT c = lds_vec<T>(pl, pw);
+        const VT pm = lds_vec<T>(s_pl[W ^ 1][n], pw);  // psi(s - 1)
+        const VT gx = q[n][W ^ 1] - pm;
         const VT gy = lds_vec<T>(pl, pyp) - lds_vec<T>(pl, pym);
         T zl = lane_from<T>(al, c[V - 1]), zr = lane_from<T>(ar, c[0]);
         zl = zl_ghost ? T(0) : zl;
@@ -1591,7 +1600,7 @@ __global__ __launch_bounds__((TY + 4) * 64) void monitor_zp_kernel(MonParams<T> 
         const VT fyp_v = lds_vec<T>(fyo, pyp), fym_v = lds_vec<T>(fyo, pym);
         const VT dx = fxs - fx[n][W];
         fx[n][W] = fxs;
-        const VT po = q[n][sm];
+        const VT po = pm;
         VT v;
 #pragma unroll
         for (int e = 0; e < V; ++e) {
@@ -1602,15 +1611,15 @@ __global__ __launch_bounds__((TY + 4) * 64) void monitor_zp_kernel(MonParams<T> 
         dz[n] = dzn;
         zst<T>(rout[n], (orb_ok[n] ? ob : BAD) + ooff, v);
         // ---- publish plane s + 1
-        lds_vec_st<T>(s_pl[W ^ 1][n], pw, q[n][sp]);
+        lds_vec_st<T>(s_pl[W ^ 1][n], pw, q[n][W ^ 1]);
       }
       } else if (yflux) {
         // y-halo rows: F_y of plane s only
 #pragma unroll
         for (int n = 0; n < NB; ++n) {
           const T* pl = s_pl[W][n];
-          const VT c = q[n][s0l];
-          const VT gx = q[n][sp] - q[n][sm];
+          const VT c = lds_vec<T>(pl, pw);
+          const VT gx = q[n][W ^ 1] - lds_vec<T>(s_pl[W ^ 1][n], pw);
           const VT gy = lds_vec<T>(pl, pyp) - lds_vec<T>(pl, pym);
           T zl = lane_from<T>(al, c[V - 1]), zr = lane_from<T>(ar, c[0]);
           zl = zl_ghost ? T(0) : zl;
@@ -1622,15 +1631,15 @@ __global__ __launch_bounds__((TY + 4) * 64) void monitor_zp_kernel(MonParams<T> 
             f2[e] = flux3(gc[G][3][e], gx[e], gc[G][1][e], gy[e], gc[G][5][e], gz);
           }
           lds_vec_st<T>(s_fy[W][n], pw, f2);
-          lds_vec_st<T>(s_pl[W ^ 1][n], pw, q[n][sp]);
+          lds_vec_st<T>(s_pl[W ^ 1][n], pw, q[n][W ^ 1]);
         }
       } else {
 #pragma unroll
-        for (int n = 0; n < NB; ++n) lds_vec_st<T>(s_pl[W ^ 1][n], pw, q[n][sp]);
+        for (int n = 0; n < NB; ++n) lds_vec_st<T>(s_pl[W ^ 1][n], pw, q[n][W ^ 1]);
       }
       fetch_geom(s + GD, G);
       fetch_out(s);
-      fetch_psi(s + 3, sm);
+      fetch_psi(s + 3, W ^ 1);
       __syncthreads();
     }
   }
@@ -1688,6 +1697,7 @@ int launch_monitor(const MonParams<T>& p, int closure, int transpose, int tile, 
       case 182: return launch_monitor_zp_t<T, 2, 2, 8>(p, s);
       case 122: return launch_monitor_zp_t<T, 2, 1, 12>(p, s);
       case 43: return launch_monitor_zp_t<T, 3, 1, 4>(p, s);
+      case 64: return launch_monitor_zp_t<T, 4, 1, 6>(p, s);
       default: return IG_EUNSUPPORTED;
     }
   }

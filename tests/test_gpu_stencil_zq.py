@@ -23,7 +23,7 @@ pytestmark = pytest.mark.gpu
 
 ZQ = 9000000  # tile code forcing the z-vectorised register-queue kernel (+ orbitals per workgroup)
 ZP = 9200000  # tile code forcing the pipelined full-z kernel (+ rows x 10 + orbitals, + 100: deeper prefetch)
-ZP_VARIANTS = (81, 82, 83, 182, 122, 43)
+ZP_VARIANTS = (81, 82, 83, 182, 122, 43, 64)
 GENERAL = 23208  # the general kernel: 2 orbitals, 8 x 32 tiles
 
 
