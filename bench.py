@@ -290,12 +290,14 @@ def products_leg(n=224, m=192, k=48, reps=3):
     """MFMA utilisation of the eigensolver's two product shapes at the benzene block size
     (N = n^3, working subspace m, k states), timed with HIP events on the launch stream:
     the basis rotation Z' = Q [Z; R] (2 m (m + k) N flop) on the library's own MFMA kernel
-    (ig_rotate_*) and, for comparison, on the vendor GEMM (FP32 in 2^20-column panels), and
-    the Rayleigh-Ritz Gram block [Z; R] [R; HR]^T (2 (m + k) 2k N flop), in FP32 and FP64,
-    on seeded random blocks."""
+    (ig_rotate_*) and the Rayleigh-Ritz Gram block [Z; R] [R; HR]^T (2 (m + k) 2k N flop) on
+    the library's split-K MFMA kernel (ig_gemm_nt_*), each also on the vendor GEMMs for
+    comparison (rotation: FP32 in 2^20-column panels; Gram: split-K strided-batched), in
+    FP32 and FP64, on seeded random blocks."""
     import torch
 
-    from isogriddft_amd.eigensolver import MM_PANEL, _gemm_nt_split, _rotate_kernel
+    from isogriddft_amd import eigensolver as es
+    from isogriddft_amd.eigensolver import MM_PANEL, _gemm_nt_kernel, _gemm_nt_split, _rotate_kernel
 
     N = n ** 3
     out = {"shapes": f"N = {n}^3, rotation ({m} x {m + k}) x ({m + k} x N), Gram ({m + k} x N) x (N x {2 * k})"}
@@ -314,9 +316,21 @@ def products_leg(n=224, m=192, k=48, reps=3):
             if not _rotate_kernel(Q, B, C):
                 raise RuntimeError("the MFMA rotation kernel declined the benchmark shape")
 
+        def own_gram():
+            if _gemm_nt_kernel(B, B[:2 * k]) is None:
+                raise RuntimeError("the split-K MFMA product declined the benchmark shape")
+
+        def vendor_gram():
+            es.GEMM_NT_KERNEL = False
+            try:
+                _gemm_nt_split(B, B[:2 * k])
+            finally:
+                es.GEMM_NT_KERNEL = True
+
         legs = (("rotation", own_rotation, 2.0 * m * (m + k) * N),
                 ("rotation_vendor", vendor_rotation, 2.0 * m * (m + k) * N),
-                ("gram", lambda: _gemm_nt_split(B, B[:2 * k]), 2.0 * (m + k) * 2 * k * N))
+                ("gram", own_gram, 2.0 * (m + k) * 2 * k * N),
+                ("gram_vendor", vendor_gram, 2.0 * (m + k) * 2 * k * N))
         for what, fn, flop in legs:
             fn()
             torch.cuda.synchronize()

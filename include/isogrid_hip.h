@@ -139,6 +139,22 @@ int ig_rotate_f32(const float* Q, int ldq, const float* X, long long ldx, float*
 int ig_rotate_f64(const double* Q, int ldq, const double* X, long long ldx, double* Y, long long ldy, int M,
                   int K, long long N, int ct, hipStream_t stream);
 
+/* Doubles of `work` that ig_gemm_nt_f32/f64 need for A (m x K) and B (n x K): one FP64
+ * (m x n) partial per K panel. */
+long long ig_gemm_nt_work_size(int m, int n, long long K);
+
+/* C[m x n] (FP64, row stride ldc) = A B^T on MFMA (v_mfma_f32_16x16x4f32 /
+ * v_mfma_f64_16x16x4f64), split-K over the long axis K with a deterministic FP64 sum of
+ * the panel partials: A (m x K, row stride lda) and B (n x K, row stride ldb) row-major
+ * with unit column stride and 16-byte aligned rows (-2 otherwise), `work` at least
+ * ig_gemm_nt_work_size(m, n, K) doubles.  Replaces the projected-matrix products of the
+ * block iteration: ops/reductions_jax.py:157-159 (Gram blocks), ks/eigensolver_jax.py:
+ * 108-146 (basis overlap for the orthonormalisation), ks/eigensolver.py:246-272. */
+int ig_gemm_nt_f32(const float* A, long long lda, int m, const float* B, long long ldb, int n, long long K,
+                   double* work, long long work_size, double* C, long long ldc, hipStream_t stream);
+int ig_gemm_nt_f64(const double* A, long long lda, int m, const double* B, long long ldb, int n, long long K,
+                   double* work, long long work_size, double* C, long long ldc, hipStream_t stream);
+
 
 /* ------------------------------------------------------------------ physics fields */
 

@@ -40,6 +40,9 @@ SIGNATURES = {
     "ig_gram_f64": [P, L, I, P, L, I, P, L, P, I, P, L, P],
     "ig_rotate_f32": [P, I, P, L, P, L, I, I, L, I, P],
     "ig_rotate_f64": [P, I, P, L, P, L, I, I, L, I, P],
+    "ig_gemm_nt_work_size": [I, I, L],
+    "ig_gemm_nt_f32": [P, L, I, P, L, I, L, P, L, P, L, P],
+    "ig_gemm_nt_f64": [P, L, I, P, L, I, L, P, L, P, L, P],
     "ig_physics_work_size": [L],
     "ig_vloc_gth_f64": [P, L, I, P, P, I, P],
     "ig_vloc_gth_grad_f64": [P, P, L, I, P, P, P, P],
@@ -89,7 +92,7 @@ SIGNATURES = {
     "ig_stream_copy_f32": [P, P, L, P],
     "ig_stream_read_f32": [P, L, P, P],
 }
-_RESTYPE = {"ig_gram_work_size": L, "ig_knn_work_bytes": L}
+_RESTYPE = {"ig_gram_work_size": L, "ig_knn_work_bytes": L, "ig_gemm_nt_work_size": L}
 
 _lib = None
 

@@ -64,6 +64,14 @@ FP32_STALL_CEILING = 100.0
 
 # widest column block handed to one FP32 library GEMM
 MM_PANEL = 1 << 20
+# Gram blocks of the iteration on the library's split-K MFMA kernel (False: vendor GEMMs
+# only).  The kernel streams the grid axis in 128-byte slices with one barrier each, so it
+# pays for the large blocks, where it keeps the matrix cores busy (N = 224^3, 240 x 96:
+# FP64 10.2 ms against 15.0 ms for the split-K batched vendor GEMM, FP32 5.1 against 5.1),
+# and not for short ones (48 x 48: FP64 2.9 against 1.7 ms; 3 x 3: 2.2 against 1.3 ms):
+# rows of A from which it is used, per precision.
+GEMM_NT_KERNEL = True
+GEMM_NT_MIN_ROWS = {torch.float64: (96, 1), torch.float32: (192, 64)}
 # rows of Q the MFMA rotation kernel accumulates in registers
 ROTATE_MAX_ROWS = 256
 
@@ -241,16 +249,43 @@ NT_PANELS = 256
 NT_MIN_PANEL = 4096
 
 
+def _gemm_nt_kernel(A, B):
+    """A B^T (FP64) on the library's split-K MFMA kernel (ig_gemm_nt_*), or None (nothing
+    launched) when the kernel does not take the operands (16-byte aligned rows)."""
+    a, N = A.shape
+    b = B.shape[0]
+    lib = _lib.load()
+    wsize = lib.ig_gemm_nt_work_size(a, b, N)
+    if wsize <= 0:
+        return None
+    work = torch.empty(wsize, dtype=torch.float64, device=A.device)
+    G = torch.empty((a, b), dtype=torch.float64, device=A.device)
+    sfx = "f32" if A.dtype == torch.float32 else "f64"
+    st = getattr(lib, f"ig_gemm_nt_{sfx}")(_lib.ptr(A), A.stride(0), a, _lib.ptr(B), B.stride(0), b, N,
+                                          _lib.ptr(work), wsize, _lib.ptr(G), b, _lib.stream_handle())
+    if st == -3:
+        raise RuntimeError(f"ig_gemm_nt_{sfx}: HIP launch failed")
+    return G if st == 0 else None
+
+
 def _gemm_nt_split(A, B):
-    """A B^T (FP64 result) for A (a, N), B (b, N) with unit column stride.  The grid axis
-    is cut into P equal panels that run as one strided-batched library GEMM (a single
-    GEMM with K = N this long keeps only a few workgroups busy), the remainder as one
-    more product, and the partials are added in FP64 in a fixed order.  In FP32 each
-    panel accumulates at most N / P products before the FP64 sum."""
+    """A B^T (FP64 result) for A (a, N), B (b, N) with unit column stride.  Large device
+    blocks (GEMM_NT_MIN_ROWS) go to the library's split-K MFMA kernel (ig_gemm_nt_*: every
+    panel of the grid axis read once, all a x b partials of a panel in MFMA accumulators,
+    FP64 sum of the partials in a fixed order).  Otherwise the grid axis is cut into P equal panels that run as one
+    strided-batched library GEMM (a single GEMM with K = N this long keeps only a few
+    workgroups busy), the remainder as one more product, and the partials are added in
+    FP64 in a fixed order.  In FP32 each panel accumulates at most N / P products before
+    the FP64 sum."""
     a, N = A.shape
     b = B.shape[0]
     if A.stride(1) != 1 or B.stride(1) != 1:
         raise ValueError("Gram operands need unit column stride")
+    if (A.is_cuda and A.dtype == B.dtype and A.dtype in GEMM_NT_MIN_ROWS and GEMM_NT_KERNEL
+            and a >= GEMM_NT_MIN_ROWS[A.dtype][0] and b >= GEMM_NT_MIN_ROWS[A.dtype][1]):
+        G = _gemm_nt_kernel(A, B)
+        if G is not None:
+            return G
     kp = max(NT_MIN_PANEL, -(-N // NT_PANELS))
     kp = -(-kp // 64) * 64
     P = N // kp
