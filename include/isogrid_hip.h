@@ -110,6 +110,47 @@ int ig_bicgstab_x_f32(long long n, float* x, const float* phat, const float* sha
                       const float* t, float* r, const float* rhat, double* sc, double* work,
                       hipStream_t stream);
 
+/* Multigrid V-cycle of the monitor-grid Poisson preconditioner (the right preconditioner
+   of the BiCGSTAB above).  Replaces nothing one to one: the reference runs the same
+   interior solve unpreconditioned (poisson/open_boundary.py:1017-1064) or with a line
+   preconditioner on its JAX path (poisson/poisson_jax.py:411-579).  Coarse spaces follow
+   the parity split of the 2h-wide stencil: level 1 aggregates the interior nodes of one
+   parity in a 4x4x4 block (fine node i -> parity p = i & 1, aggregate (i - p) >> 2 per
+   axis), deeper levels 2x2x2 points of one parity.  A level vector of extents (cx, cy,
+   cz) is laid out [p][Sx][Sy][Sz] (p = 4 px + 2 py + pz); its operator is coef[7][m],
+   m = 8 cx cy cz, {diagonal, -x, +x, -y, +y, -z, +z}.  Level-1 extents are
+   ceil(n / 4), each further level ceil(c / 2); other extents return -1. */
+/* bc = sum over the interior children of (b - t) (t may be NULL: sum of b). */
+int ig_mg_fine_restrict_f64(const double* b, const double* t, int nx, int ny, int nz, double* bc, int cx,
+                            int cy, int cz, hipStream_t stream);
+int ig_mg_fine_restrict_f32(const float* b, const float* t, int nx, int ny, int nz, float* bc, int cx, int cy,
+                            int cz, hipStream_t stream);
+/* x[i] += ec[aggregate of i] on interior nodes. */
+int ig_mg_fine_prolong_f64(double* x, const double* ec, int nx, int ny, int nz, int cx, int cy, int cz,
+                           hipStream_t stream);
+int ig_mg_fine_prolong_f32(float* x, const float* ec, int nx, int ny, int nz, int cx, int cy, int cz,
+                           hipStream_t stream);
+/* x += dw (b - t) over n points (fine damped Jacobi, dw = omega / diagonal, 0 on faces). */
+int ig_mg_smooth_f64(double* x, const double* b, const double* t, const double* dw, long long n,
+                     hipStream_t stream);
+int ig_mg_smooth_f32(float* x, const float* b, const float* t, const float* dw, long long n, hipStream_t stream);
+/* xo = xi + omega (b - A xi) / diag on one level (xi NULL: xo = omega b / diag; xo != xi). */
+int ig_mg7_jacobi_f64(double* xo, const double* xi, const double* b, const double* coef, double omega, int cx,
+                      int cy, int cz, hipStream_t stream);
+int ig_mg7_jacobi_f32(float* xo, const float* xi, const float* b, const float* coef, double omega, int cx, int cy,
+                      int cz, hipStream_t stream);
+/* bc (level extents d) = sum over the 2x2x2 children of (b - A x) on level extents c
+   (x NULL: sum of b). */
+int ig_mg7_restrict_f64(double* bc, const double* b, const double* x, const double* coef, int cx, int cy, int cz,
+                        int dx, int dy, int dz, hipStream_t stream);
+int ig_mg7_restrict_f32(float* bc, const float* b, const float* x, const float* coef, int cx, int cy, int cz,
+                        int dx, int dy, int dz, hipStream_t stream);
+/* x[p][s] += ec[p][s >> 1]. */
+int ig_mg7_prolong_f64(double* x, const double* ec, int cx, int cy, int cz, int dx, int dy, int dz,
+                       hipStream_t stream);
+int ig_mg7_prolong_f32(float* x, const float* ec, int cx, int cy, int cz, int dx, int dy, int dz,
+                       hipStream_t stream);
+
 /* ------------------------------------------------------------------ Gram products */
 
 /* Doubles of `work` required by ig_gram_* for an (m x n) product over N points. */

@@ -35,6 +35,18 @@ SIGNATURES = {
     "ig_bicgstab_s_f32": [L, P, P, P, P, P, P, P],
     "ig_bicgstab_x_f64": [L, P, P, P, P, P, P, P, P, P, P],
     "ig_bicgstab_x_f32": [L, P, P, P, P, P, P, P, P, P, P],
+    "ig_mg_fine_restrict_f64": [P, P, I, I, I, P, I, I, I, P],
+    "ig_mg_fine_restrict_f32": [P, P, I, I, I, P, I, I, I, P],
+    "ig_mg_fine_prolong_f64": [P, P, I, I, I, I, I, I, P],
+    "ig_mg_fine_prolong_f32": [P, P, I, I, I, I, I, I, P],
+    "ig_mg_smooth_f64": [P, P, P, P, L, P],
+    "ig_mg_smooth_f32": [P, P, P, P, L, P],
+    "ig_mg7_jacobi_f64": [P, P, P, P, D, I, I, I, P],
+    "ig_mg7_jacobi_f32": [P, P, P, P, D, I, I, I, P],
+    "ig_mg7_restrict_f64": [P, P, P, P, I, I, I, I, I, I, P],
+    "ig_mg7_restrict_f32": [P, P, P, P, I, I, I, I, I, I, P],
+    "ig_mg7_prolong_f64": [P, P, I, I, I, I, I, I, P],
+    "ig_mg7_prolong_f32": [P, P, I, I, I, I, I, I, P],
     "ig_gram_work_size": [I, I, L],
     "ig_gram_f32": [P, L, I, P, L, I, P, L, P, I, P, L, P],
     "ig_gram_f64": [P, L, I, P, L, I, P, L, P, I, P, L, P],

@@ -38,7 +38,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, multigrid
 from .device import DeviceGrid, device_grid
 from .kinetic import apply_operator
 from .reductions import dots_into, workspace
@@ -453,13 +453,19 @@ def _matvec(dg, x, out, stream=None, transpose=False):
     out.mul_(poisson_data(dg).interior_mask(out.dtype))
 
 
-def bicgstab(dg, b, x, *, dinv=None, rtol=1e-8, maxiter=400, check_every=8, stream=None, transpose=False):
+def bicgstab(dg, b, x, *, dinv=None, rtol=1e-8, maxiter=400, check_every=8, stream=None, transpose=False,
+             mg=None):
     """Solve (-L) x = b on interior nodes (transpose: (-L)^T x = b); x (flat CUDA tensor)
     is overwritten.
 
     b and x share a dtype (float32 / float64); Krylov scalars stay FP64 on the device.
+    Preconditioner (right): the diagonal ``dinv``, or one multigrid V-cycle ``mg``
+    (multigrid.PoissonMultigrid of the forward operator): the fused p / s updates then
+    write its first damped-Jacobi sweep and ``mg.finish`` completes the cycle in place.
     Returns (iterations, converged, relative residual ||r|| / ||b||).
     """
+    if mg is not None and (transpose or dinv is not None):
+        raise ValueError("the multigrid preconditioner serves the forward operator alone, without dinv")
     dt = b.dtype
     n = b.numel()
     sp = _space(n, dt, b.device)
@@ -481,6 +487,9 @@ def bicgstab(dg, b, x, *, dinv=None, rtol=1e-8, maxiter=400, check_every=8, stre
     sc[2] = 1.0
     sc[3] = 1.0
     sc[7] = bb
+    if mg is not None:
+        dinv = mg.scaled_inverse_diagonal(dt)
+        check_every = min(check_every, 2)  # a few tens of iterations: test often
     dv = _lib.ptr(dinv.to(dt) if dinv is not None and dinv.dtype != dt else dinv)
     it = 0
     rel = 1.0
@@ -489,10 +498,14 @@ def bicgstab(dg, b, x, *, dinv=None, rtol=1e-8, maxiter=400, check_every=8, stre
         it += 1
         _lib.call(f"ig_bicgstab_p_{sfx}", n, _lib.ptr(sp.r), _lib.ptr(sp.p), _lib.ptr(sp.v), dv,
                   _lib.ptr(sp.phat), _lib.ptr(sc), h)
+        if mg is not None:
+            mg.finish(sp.p, sp.phat, stream)
         _matvec(dg, sp.phat, sp.v, stream, transpose)
         dots_into(sc, [(sp.rhat, sp.v)], None, slots=(4, 0, 0), stream=stream)
         _lib.call(f"ig_bicgstab_s_{sfx}", n, _lib.ptr(sp.r), _lib.ptr(sp.v), dv, _lib.ptr(sp.s),
                   _lib.ptr(sp.shat), _lib.ptr(sc), h)
+        if mg is not None:
+            mg.finish(sp.s, sp.shat, stream)
         _matvec(dg, sp.shat, sp.t, stream, transpose)
         dots_into(sc, [(sp.t, sp.s), (sp.t, sp.t)], None, slots=(5, 6, 0), stream=stream)
         _lib.call(f"ig_bicgstab_x_{sfx}", n, _lib.ptr(x), _lib.ptr(sp.phat), _lib.ptr(sp.shat), _lib.ptr(sp.s),
@@ -550,14 +563,34 @@ def _diag_inverse(dg, pdata):
     return inv
 
 
+def poisson_multigrid(dg):
+    """The multigrid preconditioner of this grid's interior operator (built on first use,
+    then cached with the grid's Poisson data; monitor grids only)."""
+    pdata = poisson_data(dg)
+    mg = getattr(pdata, "multigrid", None)
+    if mg is None:
+        if dg.kind != "monitor":
+            raise ValueError("the multigrid preconditioner is built for monitor grids")
+        mg = multigrid.PoissonMultigrid(dg.shape, lambda x, out: _matvec(dg, x, out), dg.device)
+        pdata.multigrid = mg
+    return mg
+
+
 def solve_poisson_device(dg, rho, *, multipole_order=2, tolerance=1e-8, max_iterations=400, solver="auto",
                          mode="corrected_moments", precision="fp64", check_every=8, inner_tolerance=None,
-                         stream=None):
+                         stream=None, preconditioner="auto"):
     """Device-resident open-boundary solve.  rho: FP64 CUDA tensor (validated, >= 0).
 
+    preconditioner: "auto" (multigrid V-cycle on monitor grids, the inverse diagonal on
+    legacy grids), "multigrid" (monitor grids), "jacobi" (the inverse diagonal) or "none".
     Returns a dict: potential (FP64, grid shape), boundary (b), moments (flat, host),
     baseline (flat, host), iterations, converged, residual (relative 2-norm), method.
     """
+    if preconditioner not in ("auto", "multigrid", "jacobi", "none"):
+        raise ValueError("preconditioner must be `auto`, `multigrid`, `jacobi` or `none`; "
+                         f"received `{preconditioner}`.")
+    if preconditioner == "multigrid" and dg.kind != "monitor":
+        raise ValueError("the multigrid preconditioner is built for monitor grids")
     if solver not in _SOLVERS:
         raise ValueError(
             "The open-boundary Poisson solve supports solver=`auto`, `scipy_bicgstab`, `bicgstab` "
@@ -572,15 +605,22 @@ def solve_poisson_device(dg, rho, *, multipole_order=2, tolerance=1e-8, max_iter
     _matvec(dg, b, rhs, stream)
     rhs.neg_().add_(rho * pdata.interior, alpha=FOUR_PI)
     u = torch.zeros_like(rho)
-    dinv = pdata.dinv  # Jacobi preconditioner on legacy grids, none on monitor grids
+    mg = None
+    dinv = None
+    pre = ("multigrid" if dg.kind == "monitor" else "jacobi") if preconditioner == "auto" else preconditioner
+    if solver != "jacobi" and pre == "multigrid":
+        mg = poisson_multigrid(dg)
+    elif pre == "jacobi":
+        dinv = pdata.dinv if pdata.dinv is not None else _diag_inverse(dg, pdata)
+    tag = "_mg" if mg is not None else ""
     if solver == "jacobi":
         it, conv, res = _jacobi(dg, rhs, u, _diag_inverse(dg, pdata), tolerance, max_iterations, stream)
         method = "hip_jacobi" if conv else "hip_jacobi_maxiter"
         rel = res
     elif precision == "fp64":
         it, conv, rel = bicgstab(dg, rhs, u, dinv=dinv, rtol=tolerance, maxiter=max_iterations,
-                                 check_every=check_every, stream=stream)
-        method = "hip_bicgstab_fp64" + ("_monitor" if dg.kind == "monitor" else "")
+                                 check_every=check_every, stream=stream, mg=mg)
+        method = f"hip_bicgstab{tag}_fp64" + ("_monitor" if dg.kind == "monitor" else "")
     else:
         # FP32 BiCGSTAB corrections of the FP64 residual (defect correction).  Each inner
         # solve restarts its Krylov space, so it is asked for the whole remaining
@@ -602,7 +642,7 @@ def solve_poisson_device(dg, rho, *, multipole_order=2, tolerance=1e-8, max_iter
             inner = inner_tolerance if inner_tolerance is not None else \
                 max(0.5 * tolerance * rn / scale, MIXED_INNER_FLOOR)
             k, _, _ = bicgstab(dg, r32, d32, dinv=dinv32, rtol=inner, maxiter=max_iterations - it,
-                               check_every=check_every, stream=stream)
+                               check_every=check_every, stream=stream, mg=mg)
             it += k
             u.add_(d32.double(), alpha=scale)
             _matvec(dg, u, tmp, stream)
@@ -614,7 +654,7 @@ def solve_poisson_device(dg, rho, *, multipole_order=2, tolerance=1e-8, max_iter
             # stagnation at the FP32 floor) ends the loop; so does a non-finite residual
             if outer > 64 or not math.isfinite(rel) or (not conv and rel >= prev):
                 break
-        method = "hip_bicgstab_mixed_fp32_fp64" + ("_monitor" if dg.kind == "monitor" else "")
+        method = f"hip_bicgstab{tag}_mixed_fp32_fp64" + ("_monitor" if dg.kind == "monitor" else "")
     potential = b + u
     # interior residual max |(-L) u - rhs|
     res = torch.empty_like(rho)
