@@ -230,6 +230,11 @@ def solver_leg(geom, dg, cfg, eig_tol, order=2, block=None):
     rho = configs.build_synthetic_density(geom, cfg, device="cuda").to(torch.float64)
     out = {}
     pots = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    poisson.poisson_multigrid(dg)
+    torch.cuda.synchronize()
+    out["poisson_mg_setup_s"] = round(time.perf_counter() - t0, 4)
     for prec in ("fp64", "mixed"):
         kw = dict(multipole_order=order, tolerance=1e-8, precision=prec, max_iterations=4000)
         poisson.solve_poisson_device(dg, rho, **kw)  # warm
@@ -244,6 +249,13 @@ def solver_leg(geom, dg, cfg, eig_tol, order=2, block=None):
         print(f"[bench] poisson {prec}: {out[f'poisson_{prec}_s']} s", file=sys.stderr, flush=True)
     out["poisson_mixed_vs_fp64_max_rel"] = float((pots["mixed"] - pots["fp64"]).abs().max()
                                                  / pots["fp64"].abs().max())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sol = poisson.solve_poisson_device(dg, rho, multipole_order=order, tolerance=1e-8, max_iterations=4000,
+                                       preconditioner="none")
+    torch.cuda.synchronize()
+    out["poisson_fp64_unpreconditioned_s"] = round(time.perf_counter() - t0, 4)
+    out["poisson_fp64_unpreconditioned_iterations"] = sol["iterations"]
     # a neutral molecule-like potential: two Z = 3 soft-Coulomb centres plus the Hartree
     # potential of the 6-electron synthetic density (trial_fix: exactly W-self-adjoint)
     xyz = dg.xyz()
@@ -278,7 +290,7 @@ def solver_leg(geom, dg, cfg, eig_tol, order=2, block=None):
     out["eigensolve_plus_poisson_mixed_s"] = round(out["eig_mixed_s"] + out["poisson_mixed_s"], 4)
     out["mixed_speedup"] = round(out["eigensolve_plus_poisson_fp64_s"] / out["eigensolve_plus_poisson_mixed_s"], 3)
     out["workload"] = (f"{dg.shape[0]}^3 grid, Poisson: {cfg.n_charges} Gaussian charges, order-{order} multipoles, "
-                       f"rtol 1e-8; eigensolve: k={k}, working subspace {m}, residual tol {eig_tol}, "
+                       f"rtol 1e-8, multigrid-preconditioned BiCGSTAB; eigensolve: k={k}, working subspace {m}, residual tol {eig_tol}, "
                        "trial_fix closure, V = two Z=3 soft-Coulomb centres + v_H")
     return out
 
@@ -362,8 +374,9 @@ def molecule_leg(name):
 
     kw = MOLECULE_LEGS[name]
     r = workloads.run(name, kw["n"], kw["k"], kw["block"], kw["tol"])
-    keys = ("workload", "k", "occupied", "grid_s", "poisson_fp64_s", "poisson_mixed_s", "poisson_fp64_iterations",
-            "poisson_mixed_iterations", "poisson_mixed_vs_fp64_max_rel", "eig_fp64_s", "eig_mixed_s",
+    keys = ("workload", "k", "occupied", "grid_s", "poisson_mg_setup_s", "poisson_fp64_s", "poisson_mixed_s",
+            "poisson_fp64_iterations", "poisson_mixed_iterations", "poisson_mixed_vs_fp64_max_rel",
+            "poisson_fp64_unpreconditioned_s", "poisson_fp64_unpreconditioned_iterations", "eig_fp64_s", "eig_mixed_s",
             "eig_fp64_iterations", "eig_mixed_iterations", "eig_fp32_iterations", "eig_fp64_converged",
             "eig_mixed_converged", "eig_fp64_phase_s", "eig_mixed_phase_s", "eig_fp64_active_fraction",
             "eig_mixed_active_fraction", "eig_mixed_vs_fp64_max_abs_Ha",

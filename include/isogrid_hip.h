@@ -145,6 +145,14 @@ int ig_mg7_restrict_f64(double* bc, const double* b, const double* x, const doub
                         int dx, int dy, int dz, hipStream_t stream);
 int ig_mg7_restrict_f32(float* bc, const float* b, const float* x, const float* coef, int cx, int cy, int cz,
                         int dx, int dy, int dz, hipStream_t stream);
+/* Hierarchy set-up (FP64): the 28 probe vectors that read the fine same-parity entries
+   off the operator.  Probe q (0..27) = 7 pair + colour holds the interior nodes of parity
+   class c = 4 px + 2 py + pz with min(c, 7 - c) = pair and ((ix >> 1) + 2 (iy >> 1) +
+   3 (iz >> 1)) mod 7 = colour; probe_read stores out[i] into ent[k][i] (ent: 7 fields of
+   nx ny nz) for every interior node i whose offset-k partner (0, -+2 e_x, -+2 e_y,
+   -+2 e_z) is in probe q. */
+int ig_mg_probe_vector_f64(double* e, int q, int nx, int ny, int nz, hipStream_t stream);
+int ig_mg_probe_read_f64(double* ent, const double* out, int q, int nx, int ny, int nz, hipStream_t stream);
 /* x[p][s] += ec[p][s >> 1]. */
 int ig_mg7_prolong_f64(double* x, const double* ec, int cx, int cy, int cz, int dx, int dy, int dz,
                        hipStream_t stream);

@@ -47,6 +47,8 @@ SIGNATURES = {
     "ig_mg7_restrict_f32": [P, P, P, P, I, I, I, I, I, I, P],
     "ig_mg7_prolong_f64": [P, P, I, I, I, I, I, I, P],
     "ig_mg7_prolong_f32": [P, P, I, I, I, I, I, I, P],
+    "ig_mg_probe_vector_f64": [P, I, I, I, I, P],
+    "ig_mg_probe_read_f64": [P, P, I, I, I, I, P],
     "ig_gram_work_size": [I, I, L],
     "ig_gram_f32": [P, L, I, P, L, I, P, L, P, I, P, L, P],
     "ig_gram_f64": [P, L, I, P, L, I, P, L, P, I, P, L, P],

@@ -22,6 +22,12 @@
 //   mg7_jacobi      xo = xi + omega (b - A xi) / diag   (xi null: xo = omega b / diag)
 //   mg7_restrict    bc[p][S] = sum over children 2S + d of (b - A x)  (x null: sum of b)
 //   mg7_prolong     x[p][s] += ec[p][s >> 1]
+// Probing (hierarchy set-up, FP64): node i of parity class cls = 4 px + 2 py + pz lies on
+// sub-lattice point s = i >> 1 with colour (sx + 2 sy + 3 sz) mod 7; classes cls and
+// 7 - cls share a probe (pair = min of the two), so probe q = 7 pair + colour.
+//   probe_vector    e[i] = 1 on the interior nodes of probe q, else 0
+//   probe_read      ent[k][i] = out[i] where probe q holds node i + o_k (o_k the k-th
+//                   7-point offset in fine nodes: 0, -+2 e_x, -+2 e_y, -+2 e_z)
 // The solve these kernels precondition replaces the reference's unpreconditioned
 // interior BiCGSTAB (poisson/open_boundary.py:1017-1064) and the line-preconditioned CG of
 // its JAX path (poisson/poisson_jax.py:411-579); the discrete solution is the same.
@@ -169,6 +175,41 @@ __global__ __launch_bounds__(IG_THREADS) void mg7_prolong(T* __restrict__ x, con
   }
 }
 
+__device__ __forceinline__ bool probe_node(long long i, int nx, int ny, int nz, int& pair, int& col) {
+  const int iz = (int)(i % nz);
+  const int iy = (int)((i / nz) % ny);
+  const int ix = (int)(i / ((long long)nz * ny));
+  if (ix < 1 || ix > nx - 2 || iy < 1 || iy > ny - 2 || iz < 1 || iz > nz - 2) return false;
+  const int cls = 4 * (ix & 1) + 2 * (iy & 1) + (iz & 1);
+  pair = cls >= 4 ? 7 - cls : cls;
+  col = ((ix >> 1) + 2 * (iy >> 1) + 3 * (iz >> 1)) % 7;
+  return true;
+}
+
+__global__ __launch_bounds__(IG_THREADS) void probe_vector(double* __restrict__ e, int q, int nx, int ny, int nz) {
+  const long long n = (long long)nx * ny * nz;
+  for (long long i = (long long)blockIdx.x * IG_THREADS + threadIdx.x; i < n;
+       i += (long long)gridDim.x * IG_THREADS) {
+    int pair, col;
+    e[i] = (probe_node(i, nx, ny, nz, pair, col) && 7 * pair + col == q) ? 1.0 : 0.0;
+  }
+}
+
+__global__ __launch_bounds__(IG_THREADS) void probe_read(double* __restrict__ ent, const double* __restrict__ out,
+                                                         int q, int nx, int ny, int nz) {
+  // colour shift of the 7-point offsets (o_x + 2 o_y + 3 o_z) / 2, kept in [0, 7)
+  const int shift[7] = {0, 6, 1, 5, 2, 4, 3};
+  const long long n = (long long)nx * ny * nz;
+  for (long long i = (long long)blockIdx.x * IG_THREADS + threadIdx.x; i < n;
+       i += (long long)gridDim.x * IG_THREADS) {
+    int pair, col;
+    if (!probe_node(i, nx, ny, nz, pair, col) || pair != q / 7) continue;
+#pragma unroll
+    for (int k = 0; k < 7; ++k)
+      if ((col + shift[k]) % 7 == q % 7) ent[(long long)k * n + i] = out[i];
+  }
+}
+
 inline unsigned grid_for(long long n) {
   long long g = (n + IG_THREADS - 1) / IG_THREADS;
   if (g > 8192) g = 8192;
@@ -237,9 +278,31 @@ int run_prolong(T* x, const T* ec, int cx, int cy, int cz, int dx, int dy, int d
   return IG_OK;
 }
 
+int run_probe_vector(double* e, int q, int nx, int ny, int nz, hipStream_t s) {
+  if (!e || q < 0 || q >= 28 || !dims_ok(nx, ny, nz)) return IG_EINVAL;
+  hipLaunchKernelGGL(probe_vector, dim3(grid_for((long long)nx * ny * nz)), dim3(IG_THREADS), 0, s, e, q, nx, ny, nz);
+  IG_CHECK_LAUNCH();
+  return IG_OK;
+}
+
+int run_probe_read(double* ent, const double* out, int q, int nx, int ny, int nz, hipStream_t s) {
+  if (!ent || !out || q < 0 || q >= 28 || !dims_ok(nx, ny, nz)) return IG_EINVAL;
+  hipLaunchKernelGGL(probe_read, dim3(grid_for((long long)nx * ny * nz)), dim3(IG_THREADS), 0, s, ent, out, q, nx,
+                     ny, nz);
+  IG_CHECK_LAUNCH();
+  return IG_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int ig_mg_probe_vector_f64(double* e, int q, int nx, int ny, int nz, hipStream_t s) {
+  return run_probe_vector(e, q, nx, ny, nz, s);
+}
+int ig_mg_probe_read_f64(double* ent, const double* out, int q, int nx, int ny, int nz, hipStream_t s) {
+  return run_probe_read(ent, out, q, nx, ny, nz, s);
+}
 
 int ig_mg_fine_restrict_f64(const double* b, const double* t, int nx, int ny, int nz, double* bc, int cx, int cy,
                             int cz, hipStream_t s) {

@@ -19,8 +19,7 @@ coarse spaces respect that split:
                 D^-1 A from power iterations, the coarse one COARSE_OMEGA
 
 The fine same-parity entries (diagonal and +-2 e_a couplings, closure included) are read
-off the operator kernel itself by applying it to 125 combs of stride 5 (the stencil reaches
-two nodes per axis).  The hierarchy depends on the geometry only and is built once per
+off the operator kernel itself by applying it to 28 probe vectors (``probe_keys``).  The hierarchy depends on the geometry only and is built once per
 grid.  With it the benzene 224^3 monitor system needs a few tens of BiCGSTAB iterations
 instead of ~280, and FP32 converges as fast as FP64 down to its rounding floor.  The
 preconditioner changes the iterates, not the discrete solution the solve converges to.
@@ -48,46 +47,47 @@ POWER_ITERATIONS = 20
 OFFSETS = ((0, 0, 0), (-2, 0, 0), (2, 0, 0), (0, -2, 0), (0, 2, 0), (0, 0, -2), (0, 0, 2))
 
 
+PROBES = 28
+
+
 def probe_fine_entries(shape, matvec, device):
     """A[i, i], A[i, i -+ 2 e_a] of the interior operator (7, nx, ny, nz) FP64, read off
-    ``matvec(x, out)`` (out = A x on interior nodes, 0 on the faces) with 125 comb
-    vectors: comb c holds ones on the interior nodes with i = c (mod 5) in every axis; the
-    operator reaches two nodes per axis, so row i of A comb_c sees exactly one comb node,
-    i + o with o = (c - i) mod 5 in [-2, 2]^3, and reads A[i, i + o]."""
+    ``matvec(x, out)`` (out = A x on interior nodes, 0 on the faces) with 28 probe vectors.
+
+    A node i of parity p = i & 1 sits on sub-lattice point s = i >> 1.  Within one parity
+    class the operator couples s with s +- e_a only (the 2h-wide differences), and the
+    colour (sx + 2 sy + 3 sz) mod 7 differs between any two points of such a 7-point
+    neighbourhood.  No entry couples parities that differ in all three axes (the mixed
+    terms change two parities, the one-sided face closure one), so classes p and
+    p ^ (1, 1, 1) share a probe: probe q = 7 pair + colour holds the interior nodes of that
+    pair and colour, and row i of A probe_q sees one probe node among its nonzero columns,
+    i + o, whenever its pair matches and colour(s_i + o / 2) = colour (kernels
+    ig_mg_probe_vector_f64 / ig_mg_probe_read_f64)."""
+    if torch.device(device).type != "cuda":
+        raise ValueError("probing runs the device kernels: device must be a GPU")
     nx, ny, nz = shape
-    ent = torch.zeros((7, nx, ny, nz), dtype=torch.float64, device=device)
-    e = torch.zeros(shape, dtype=torch.float64, device=device)
+    ent = torch.zeros((7,) + tuple(shape), dtype=torch.float64, device=device)
+    e = torch.empty(shape, dtype=torch.float64, device=device)
     out = torch.empty(shape, dtype=torch.float64, device=device)
-    for cx in range(5):
-        for cy in range(5):
-            for cz in range(5):
-                e.zero_()
-                e[cx::5, cy::5, cz::5] = 1.0
-                e[0] = 0.0
-                e[-1] = 0.0
-                e[:, 0] = 0.0
-                e[:, -1] = 0.0
-                e[:, :, 0] = 0.0
-                e[:, :, -1] = 0.0
-                matvec(e.view(-1), out.view(-1))
-                for k, (ox, oy, oz) in enumerate(OFFSETS):
-                    sx, sy, sz = (cx - ox) % 5, (cy - oy) % 5, (cz - oz) % 5
-                    ent[k, sx::5, sy::5, sz::5] = out[sx::5, sy::5, sz::5]
+    h = _lib.stream_handle()
+    for q in range(PROBES):
+        _lib.call("ig_mg_probe_vector_f64", _lib.ptr(e), q, nx, ny, nz, h)
+        matvec(e.view(-1), out.view(-1))
+        _lib.call("ig_mg_probe_read_f64", _lib.ptr(ent), _lib.ptr(out), q, nx, ny, nz, h)
     return ent
 
 
 def _galerkin(ent):
     """Same-parity Galerkin sums over aggregates of two children per axis.
 
-    ent: (7, ..., ax, 2, ay, 2, az, 2) entries of the finer level viewed with the child
-    offset e (0 / 1) after each aggregate index; returns (7, ..., ax, ay, az).  A coupling
-    towards +a from a child with e_a = 0 stays inside the aggregate (diagonal), one from
-    e_a = 1 goes to aggregate + 1; towards -a the other way round."""
-    lead = ent.dim() - 7
-    kids = (lead + 1, lead + 3, lead + 5)
+    ent: (7, 2, 2, 2, *rest) entries of the finer level with the child offsets (ex, ey, ez)
+    (0 / 1) leading; returns (7, *rest) indexed by aggregate.  A coupling towards +a from
+    a child with e_a = 0 stays inside the aggregate (diagonal), one from e_a = 1 goes to
+    aggregate + 1; towards -a the other way round."""
+    kids = (0, 1, 2)
 
     def part(t, axis, e):
-        return t.narrow(lead + 2 * axis + 1, e, 1).sum(dim=kids)
+        return t.narrow(axis, e, 1).sum(dim=kids)
 
     d, xm, xp, ym, yp, zm, zp = ent
     diag = d.sum(dim=kids)
@@ -100,14 +100,14 @@ def _galerkin(ent):
 
 def level_one(ent):
     """Level-1 7-point operators (7, 8, cx, cy, cz) from the fine entries (7, nx, ny, nz):
-    fine node i = p + 4 S + 2 e (per axis) belongs to aggregate S of parity p.  Raw sums:
+    fine node i = 4 S + 2 e + p (per axis) belongs to aggregate S of parity p.  Raw sums:
     aggregates without interior nodes have a zero diagonal."""
     _, nx, ny, nz = ent.shape
     c = [(n + 3) // 4 for n in (nx, ny, nz)]
     pad = torch.zeros((7, 4 * c[0], 4 * c[1], 4 * c[2]), dtype=ent.dtype, device=ent.device)
     pad[:, :nx, :ny, :nz] = ent
-    # (7, Sx, ex, px, Sy, ey, py, Sz, ez, pz) -> (7, px, py, pz, Sx, ex, Sy, ey, Sz, ez)
-    v = pad.view(7, c[0], 2, 2, c[1], 2, 2, c[2], 2, 2).permute(0, 3, 6, 9, 1, 2, 4, 5, 7, 8)
+    # (7, Sx, ex, px, Sy, ey, py, Sz, ez, pz) -> (7, ex, ey, ez, px, py, pz, Sx, Sy, Sz), one copy
+    v = pad.view(7, c[0], 2, 2, c[1], 2, 2, c[2], 2, 2).permute(0, 2, 5, 8, 3, 6, 9, 1, 4, 7).contiguous()
     return _galerkin(v).reshape(7, 8, *c).contiguous()
 
 
@@ -118,7 +118,9 @@ def next_level(coef):
     d = [(n + 1) // 2 for n in (cx, cy, cz)]
     pad = torch.zeros((7, 8, 2 * d[0], 2 * d[1], 2 * d[2]), dtype=coef.dtype, device=coef.device)
     pad[:, :, :cx, :cy, :cz] = coef
-    return _galerkin(pad.view(7, 8, d[0], 2, d[1], 2, d[2], 2)).contiguous()
+    # (7, p, Sx, ex, Sy, ey, Sz, ez) -> (7, ex, ey, ez, p, Sx, Sy, Sz)
+    v = pad.view(7, 8, d[0], 2, d[1], 2, d[2], 2).permute(0, 3, 5, 7, 1, 2, 4, 6).contiguous()
+    return _galerkin(v).contiguous()
 
 
 def finished(coef):
@@ -166,7 +168,8 @@ class PoissonMultigrid:
     """V-cycle preconditioner M ~ A^-1 of the interior monitor Poisson operator.
 
     ``matvec(x, out)`` applies A (the stencil kernel, interior rows, zero faces) in the
-    dtype of x; the hierarchy is FP64 and cast once per dtype for the cycle."""
+    dtype of x; the hierarchy is FP64 and cast once per dtype for the cycle.  ``entries``
+    (7, nx, ny, nz) skips the probing (the fine entries from elsewhere)."""
 
     def __init__(self, shape, matvec, device, entries=None):
         self.shape = tuple(shape)
@@ -182,17 +185,16 @@ class PoissonMultigrid:
         self._bufs = {}
 
     def _spectral_radius(self):
+        """Power iterations on D^-1 A from a seeded random interior vector (one host read)."""
         g = torch.Generator(device=self.device).manual_seed(1234)
         v = torch.randn(self.n, generator=g, dtype=torch.float64, device=self.device) * (self.dinv64 != 0)
         w = torch.empty_like(v)
-        lam = 0.0
         for _ in range(POWER_ITERATIONS):
             v /= torch.linalg.vector_norm(v)
             self.matvec(v, w)
             w *= self.dinv64
-            lam = float(torch.linalg.vector_norm(w).item())
             v, w = w, v
-        return lam
+        return float(torch.linalg.vector_norm(v).item())
 
     def buffers(self, dtype):
         """Per-dtype coefficients and work vectors (allocated once)."""

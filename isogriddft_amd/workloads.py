@@ -113,6 +113,12 @@ def run(name="benzene", n=224, k=None, block=None, tol=3e-4, multipole_order=4, 
     torch.cuda.synchronize()
     out["potential_setup_s"] = round(time.perf_counter() - t0, 3)
     pots = {}
+    # the multigrid preconditioner of this grid's Poisson operator (built once per grid)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    poisson.poisson_multigrid(dg)
+    torch.cuda.synchronize()
+    out["poisson_mg_setup_s"] = round(time.perf_counter() - t0, 4)
     for prec in ("fp64", "mixed"):
         kw = dict(multipole_order=multipole_order, tolerance=1e-8, precision=prec, max_iterations=4000)
         poisson.solve_poisson_device(dg, rho, **kw)
@@ -125,7 +131,16 @@ def run(name="benzene", n=224, k=None, block=None, tol=3e-4, multipole_order=4, 
         pots[prec] = sol["potential"].reshape(-1)
     out["poisson_mixed_vs_fp64_max_rel"] = float((pots["mixed"] - pots["fp64"]).abs().max()
                                                  / pots["fp64"].abs().max())
-    _log(f"poisson fp64 {out['poisson_fp64_s']} s, mixed {out['poisson_mixed_s']} s")
+    # the same FP64 solve without a preconditioner (the reference's iteration), for comparison
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sol = poisson.solve_poisson_device(dg, rho, multipole_order=multipole_order, tolerance=1e-8, max_iterations=4000,
+                                       preconditioner="none")
+    torch.cuda.synchronize()
+    out["poisson_fp64_unpreconditioned_s"] = round(time.perf_counter() - t0, 4)
+    out["poisson_fp64_unpreconditioned_iterations"] = sol["iterations"]
+    _log(f"poisson fp64 {out['poisson_fp64_s']} s, mixed {out['poisson_mixed_s']} s "
+         f"(unpreconditioned fp64 {out['poisson_fp64_unpreconditioned_s']} s)")
     veff = vloc + pots["fp64"] + vxc.reshape(-1)
     t0 = time.perf_counter()
     plan = pseudo.NonlocalPlan(dg, atoms)

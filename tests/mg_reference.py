@@ -119,3 +119,26 @@ def bicgstab(A, b, M=None, rtol=1e-10, maxiter=500):
         if rel <= rtol:
             return x, it, rel
     return x, maxiter, rel
+
+
+SHIFTS = (0, 6, 1, 5, 2, 4, 3)  # colour shift of the 7-point offsets (o_x + 2 o_y + 3 o_z) / 2 mod 7
+
+
+def probe_entries(A, shape):
+    """The 28-probe reading of the same-parity entries, restated with NumPy masks: probe
+    q = 7 pair + colour holds the interior nodes of parity pair (class p or 7 - p) and
+    sub-lattice colour ((ix >> 1) + 2 (iy >> 1) + 3 (iz >> 1)) mod 7; row i reads its
+    offset-k entry from the probe of its own pair and colour + SHIFTS[k]."""
+    I = np.indices(shape).reshape(3, -1)
+    cls = 4 * (I[0] & 1) + 2 * (I[1] & 1) + (I[2] & 1)
+    pair = np.where(cls >= 4, 7 - cls, cls)
+    col = ((I[0] >> 1) + 2 * (I[1] >> 1) + 3 * (I[2] >> 1)) % 7
+    inside = interior_mask(shape)
+    ent = np.zeros((7, I.shape[1]))
+    for q in range(28):
+        e = (inside & (7 * pair + col == q)).astype(np.float64)
+        out = A @ e
+        for k in range(7):
+            sel = inside & (pair == q // 7) & ((col + SHIFTS[k]) % 7 == q % 7)
+            ent[k, sel] = out[sel]
+    return ent.reshape((7,) + tuple(shape))

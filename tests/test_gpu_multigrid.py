@@ -1,8 +1,9 @@
 """GPU checks of the Poisson multigrid preconditioner (isogriddft_amd.multigrid,
 csrc/multigrid.hip).
 
-* The hierarchy probed through the HIP stencil equals the one built from the oracle's
-  assembled interior matrix (1e-12 relative: the two operators differ by rounding only).
+* The fine entries probed through the HIP stencil (28 probe vectors, ig_mg_probe_*) equal
+  the oracle's assembled interior matrix entries, and the hierarchy built from them the
+  one built from the oracle's matrix (1e-12 relative: the operators differ by rounding).
 * One V-cycle on the device (ig_mg_* / ig_mg7_* kernels and the stencil) equals the
   SciPy-sparse reference V-cycle of tests/mg_reference.py with the same hierarchy:
   FP64 within 1e-11 relative, FP32 within 2e-5 of the FP64 reference.  Shapes: a grid
@@ -39,6 +40,10 @@ def test_hierarchy_from_device_probes_matches_oracle_matrix(shape):
     print(f"case shape={shape}", flush=True)
     dg, mg = _mg(shape)
     A, _ = _system(shape)
+    want = _entries(A, shape)
+    got = multigrid.probe_fine_entries(shape, lambda x, out: poisson._matvec(dg, x, out), "cuda").cpu().numpy()
+    assert np.max(np.abs(got - want)) <= 1e-12 * np.abs(want).max()
+    assert np.array_equal(got == 0.0, want == 0.0)
     levels, inv = multigrid.build_levels(torch.from_numpy(_entries(A, shape)))
     assert len(levels) == len(mg.levels64)
     for a, b in zip(mg.levels64, levels):

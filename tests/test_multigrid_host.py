@@ -78,13 +78,14 @@ def _same_parity_seven_point(Ac, c):
 
 @pytest.mark.parametrize("shape", SHAPES)
 def test_probing_reads_the_operator_entries(shape):
+    """The 28-probe rule (the device kernels' key arithmetic, restated with NumPy masks)
+    recovers every same-parity entry exactly: no probe node other than the intended one
+    lies among a row's nonzero columns."""
+    import mg_reference as mref
+
     A, _ = _system(shape)
     ref = _entries(A, shape)
-
-    def matvec(x, out):
-        out.copy_(torch.from_numpy(A @ x.numpy()))
-
-    got = multigrid.probe_fine_entries(shape, matvec, "cpu").numpy()
+    got = mref.probe_entries(A, shape)
     assert np.array_equal(got, ref)
     # the probed set holds the whole same-parity part of A: nothing else couples i with
     # a node of its own parity
@@ -155,8 +156,13 @@ def _host_multigrid(shape):
     def matvec(x, out):
         out.copy_(torch.from_numpy(A @ x.numpy()))
 
-    mg = multigrid.PoissonMultigrid(shape, matvec, "cpu")
+    mg = multigrid.PoissonMultigrid(shape, matvec, "cpu", entries=torch.from_numpy(_entries(A, shape)))
     return A, interior, mg
+
+
+def test_device_probing_refuses_host_tensors():
+    with pytest.raises(ValueError):
+        multigrid.probe_fine_entries((8, 8, 8), lambda x, out: None, "cpu")
 
 
 def test_fine_damping_from_the_spectral_radius():
