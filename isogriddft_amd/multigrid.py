@@ -100,15 +100,33 @@ def _galerkin(ent):
 
 def level_one(ent):
     """Level-1 7-point operators (7, 8, cx, cy, cz) from the fine entries (7, nx, ny, nz):
-    fine node i = 4 S + 2 e + p (per axis) belongs to aggregate S of parity p.  Raw sums:
-    aggregates without interior nodes have a zero diagonal."""
+    fine node i = 4 S + 2 e + p (per axis) belongs to aggregate S of parity p.  Raw sums
+    (aggregates without interior nodes have a zero diagonal), accumulated slab by slab
+    (one strided slab of the fine entries per parity and child offset: no temporaries of
+    the grid's size), in a fixed order."""
     _, nx, ny, nz = ent.shape
     c = [(n + 3) // 4 for n in (nx, ny, nz)]
-    pad = torch.zeros((7, 4 * c[0], 4 * c[1], 4 * c[2]), dtype=ent.dtype, device=ent.device)
-    pad[:, :nx, :ny, :nz] = ent
-    # (7, Sx, ex, px, Sy, ey, py, Sz, ez, pz) -> (7, ex, ey, ez, px, py, pz, Sx, Sy, Sz), one copy
-    v = pad.view(7, c[0], 2, 2, c[1], 2, 2, c[2], 2, 2).permute(0, 2, 5, 8, 3, 6, 9, 1, 4, 7).contiguous()
-    return _galerkin(v).reshape(7, 8, *c).contiguous()
+    out = torch.zeros((7, 8) + tuple(c), dtype=ent.dtype, device=ent.device)
+    for p in range(8):
+        px, py, pz = p >> 2, (p >> 1) & 1, p & 1
+        for ex in (0, 1):
+            for ey in (0, 1):
+                for ez in (0, 1):
+                    slab = ent[:, px + 2 * ex::4, py + 2 * ey::4, pz + 2 * ez::4]
+                    lx, ly, lz = slab.shape[1:]
+                    if lx == 0 or ly == 0 or lz == 0:
+                        continue
+                    o = out[:, p, :lx, :ly, :lz]
+                    # diagonal: the node's own entry and its couplings that stay inside
+                    o[0] += slab[0]
+                    o[0] += slab[1 if ex else 2]
+                    o[0] += slab[3 if ey else 4]
+                    o[0] += slab[5 if ez else 6]
+                    # couplings that leave the aggregate: -a from e_a = 0, +a from e_a = 1
+                    o[2 if ex else 1] += slab[2 if ex else 1]
+                    o[4 if ey else 3] += slab[4 if ey else 3]
+                    o[6 if ez else 5] += slab[6 if ez else 5]
+    return out
 
 
 def next_level(coef):
@@ -160,8 +178,10 @@ def build_levels(ent):
     while max(raw[-1].shape[2:]) > COARSEST:
         raw.append(next_level(raw[-1]))
     levels = [finished(c) for c in raw]
-    inv = np.linalg.inv(dense_blocks(levels[-1]))
-    return levels, torch.from_numpy(inv).to(ent.device)
+    # torch's host LAPACK: a second BLAS thread pool in the process (NumPy's) would slow
+    # the eigensolver's small host solves that run later
+    inv = torch.linalg.inv(torch.from_numpy(dense_blocks(levels[-1])))
+    return levels, inv.to(ent.device)
 
 
 class PoissonMultigrid:

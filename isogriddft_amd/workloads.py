@@ -8,8 +8,10 @@ reference's CPU-runnable H2 case on its legacy grid instead, see ``run_h2_legacy
   density    a superposition of normalised atomic Gaussians carrying the GTH valence
              charges (a frozen start density; the reference starts its SCF from a
              superposition as well), spin-unpolarised
-  potential  V_loc (GTH-PADE) + v_H (open-boundary Poisson, multipole order 4) + v_xc
-             (LSDA, lda,vwn) and the separable GTH projectors
+  potential  V_loc (GTH-PADE) + v_H (open-boundary Poisson, multipole order 4, multigrid-
+             preconditioned BiCGSTAB in FP64 and FP32 + FP64 defect correction; one
+             unpreconditioned FP64 solve for comparison) + v_xc (LSDA, lda,vwn) and the
+             separable GTH projectors (``--poisson-only`` stops here)
   eigensolve the lowest k states of that fixed potential, in FP64 and in FP32 + FP64
              refinement (kinetic preconditioner, working subspace `block`)
   forces     Hellmann-Feynman forces -dE_ion/dR at the FP64 orbitals (ionic-energy adjoint)
@@ -87,7 +89,8 @@ def energy_components(dg, mol, orbitals, nocc):
             "E_II": e.ion_ion_repulsion, "E_total": e.total}
 
 
-def run(name="benzene", n=224, k=None, block=None, tol=3e-4, multipole_order=4, grid_device="cuda"):
+def run(name="benzene", n=224, k=None, block=None, tol=3e-4, multipole_order=4, grid_device="cuda",
+        poisson_only=False):
     mol, half, k_default, nocc = MOLECULES[name]
     k = k or k_default
     atoms = pseudo.case_atoms(mol)
@@ -113,8 +116,13 @@ def run(name="benzene", n=224, k=None, block=None, tol=3e-4, multipole_order=4, 
     torch.cuda.synchronize()
     out["potential_setup_s"] = round(time.perf_counter() - t0, 3)
     pots = {}
-    # the multigrid preconditioner of this grid's Poisson operator (built once per grid)
+    # per-grid set-up: the geometry data of the open-boundary solve (cells of the moment
+    # fit), then the multigrid hierarchy of its interior operator
     torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    poisson.poisson_data(dg)
+    torch.cuda.synchronize()
+    out["poisson_grid_setup_s"] = round(time.perf_counter() - t0, 4)
     t0 = time.perf_counter()
     poisson.poisson_multigrid(dg)
     torch.cuda.synchronize()
@@ -141,6 +149,8 @@ def run(name="benzene", n=224, k=None, block=None, tol=3e-4, multipole_order=4, 
     out["poisson_fp64_unpreconditioned_iterations"] = sol["iterations"]
     _log(f"poisson fp64 {out['poisson_fp64_s']} s, mixed {out['poisson_mixed_s']} s "
          f"(unpreconditioned fp64 {out['poisson_fp64_unpreconditioned_s']} s)")
+    if poisson_only:
+        return out
     veff = vloc + pots["fp64"] + vxc.reshape(-1)
     t0 = time.perf_counter()
     plan = pseudo.NonlocalPlan(dg, atoms)
@@ -252,6 +262,8 @@ def main(argv=None):
     ap.add_argument("--k", type=int, default=None, help="states (default: the molecule's k)")
     ap.add_argument("--block", type=int, default=None, help="working subspace (default 4k)")
     ap.add_argument("--tol", type=float, default=3e-4, help="eigensolver residual tolerance")
+    ap.add_argument("--poisson-only", action="store_true",
+                    help="stop after the Poisson solves (grid, density, set-up, FP64 / mixed / unpreconditioned)")
     args = ap.parse_args(argv)
     if not torch.cuda.is_available():
         print("the workloads run on the GPU only", file=sys.stderr)
@@ -259,7 +271,8 @@ def main(argv=None):
     if args.molecule == "h2_legacy":
         print(json.dumps(run_h2_legacy()), flush=True)
         return
-    print(json.dumps(run(args.molecule, args.n, args.k, args.block, args.tol)), flush=True)
+    print(json.dumps(run(args.molecule, args.n, args.k, args.block, args.tol, poisson_only=args.poisson_only)),
+          flush=True)
 
 
 if __name__ == "__main__":
