@@ -460,12 +460,13 @@ def bicgstab(dg, b, x, *, dinv=None, rtol=1e-8, maxiter=400, check_every=8, stre
 
     b and x share a dtype (float32 / float64); Krylov scalars stay FP64 on the device.
     Preconditioner (right): the diagonal ``dinv``, or one multigrid V-cycle ``mg``
-    (multigrid.PoissonMultigrid of the forward operator): the fused p / s updates then
+    (``poisson_multigrid`` of the same operator, forward or transposed): the fused p / s updates then
     write its first damped-Jacobi sweep and ``mg.finish`` completes the cycle in place.
     Returns (iterations, converged, relative residual ||r|| / ||b||).
     """
-    if mg is not None and (transpose or dinv is not None):
-        raise ValueError("the multigrid preconditioner serves the forward operator alone, without dinv")
+    if mg is not None and (dinv is not None or getattr(mg, "transpose", False) != bool(transpose)):
+        raise ValueError("the multigrid preconditioner must be built for the operator solved "
+                         "(forward or transposed) and replaces dinv")
     dt = b.dtype
     n = b.numel()
     sp = _space(n, dt, b.device)
@@ -563,16 +564,20 @@ def _diag_inverse(dg, pdata):
     return inv
 
 
-def poisson_multigrid(dg):
-    """The multigrid preconditioner of this grid's interior operator (built on first use,
-    then cached with the grid's Poisson data; monitor grids only)."""
+def poisson_multigrid(dg, transpose=False):
+    """The multigrid preconditioner of this grid's interior operator A (transpose: of A^T,
+    probed through the transposed stencil; the same parity structure), built on first use,
+    then cached with the grid's Poisson data; monitor grids only."""
     pdata = poisson_data(dg)
-    mg = getattr(pdata, "multigrid", None)
+    key = "multigrid_t" if transpose else "multigrid"
+    mg = getattr(pdata, key, None)
     if mg is None:
         if dg.kind != "monitor":
             raise ValueError("the multigrid preconditioner is built for monitor grids")
-        mg = multigrid.PoissonMultigrid(dg.shape, lambda x, out: _matvec(dg, x, out), dg.device)
-        pdata.multigrid = mg
+        mg = multigrid.PoissonMultigrid(dg.shape, lambda x, out: _matvec(dg, x, out, transpose=transpose),
+                                        dg.device)
+        mg.transpose = bool(transpose)
+        setattr(pdata, key, mg)
     return mg
 
 
@@ -705,14 +710,15 @@ def moments_adjoint_device(dg, q, order, mode="corrected_moments", out=None, str
 
 
 def poisson_adjoint_device(dg, g, multipole_order=2, mode="corrected_moments", tolerance=1e-10,
-                           max_iterations=4000, stream=None):
+                           max_iterations=4000, stream=None, preconditioner="auto"):
     """P^T g for the open-boundary Poisson map v = P rho of solve_poisson_device: the
     gradient of <g, v(rho)> with respect to rho.
 
     With b = B rho the multipole faces, S / E interior restriction / extension and
     A = S (-L) E, the forward map is v = B rho + E A^-1 S (4 pi rho + L B rho), so
         P^T g = 4 pi E y + B^T (g + L^T E y),   A^T y = S g,
-    solved by BiCGSTAB on the transposed stencil (no stored forward iterates), and
+    solved by BiCGSTAB on the transposed stencil (no stored forward iterates; preconditioner
+    as in solve_poisson_device, the multigrid one built for A^T), and
     B^T h = M^T F^T h with F the faces of each unit moment and M^T the moment adjoint.
     The legacy_split construction is not linear in rho (its correction centre is
     |d|-weighted) and has no adjoint here.  Returns (gradient, info dict)."""
@@ -720,12 +726,20 @@ def poisson_adjoint_device(dg, g, multipole_order=2, mode="corrected_moments", t
         raise NotImplementedError("the legacy_split boundary is not linear in rho; its adjoint is not built.")
     if mode not in _MODES:
         raise ValueError(f"monitor_boundary_construction_mode must be one of {_MODES}; received `{mode}`.")
+    if preconditioner not in ("auto", "multigrid", "jacobi", "none"):
+        raise ValueError("preconditioner must be `auto`, `multigrid`, `jacobi` or `none`; "
+                         f"received `{preconditioner}`.")
+    if preconditioner == "multigrid" and dg.kind != "monitor":
+        raise ValueError("the multigrid preconditioner is built for monitor grids")
     pdata = poisson_data(dg)
     g = g.reshape(-1).to(torch.float64).contiguous()
     b = g * pdata.interior_mask(torch.float64)
     y = torch.empty_like(b)
-    it, conv, rel = bicgstab(dg, b, y, dinv=pdata.dinv, rtol=tolerance, maxiter=max_iterations, stream=stream,
-                             transpose=True)
+    pre = ("multigrid" if dg.kind == "monitor" else "jacobi") if preconditioner == "auto" else preconditioner
+    mg = poisson_multigrid(dg, transpose=True) if pre == "multigrid" else None
+    dinv = (pdata.dinv if pdata.dinv is not None else _diag_inverse(dg, pdata)) if pre == "jacobi" else None
+    it, conv, rel = bicgstab(dg, b, y, dinv=dinv, rtol=tolerance, maxiter=max_iterations, stream=stream,
+                             transpose=True, mg=mg)
     lty = apply_operator(dg, y.view(dg.shape), scale=1.0, closure="production", transpose=True,
                          stream=stream).reshape(-1)
     h = g + lty

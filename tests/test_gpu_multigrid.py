@@ -13,6 +13,9 @@ csrc/multigrid.hip).
   one (FP64, 1e-9 relative at 1e-11 residual) on the 96^3 operator-level grid, in at most
   a quarter of the iterations; the mixed FP32 + FP64 solve with the FP32 V-cycle agrees
   with FP64 to 1e-8 and needs no more FP32 iterations than FP64 ones.
+* The transposed operator (the adjoint solve): its probed entries equal the oracle
+  matrix's transpose, and the adjoint gradient with the transposed hierarchy equals the
+  unpreconditioned one (1e-9 relative at 1e-12 residual) in a third of the iterations.
 """
 
 import numpy as np
@@ -96,6 +99,40 @@ def test_preconditioned_solve_matches_unpreconditioned_at_96():
     assert np.max(np.abs(pm - p1)) <= 1e-8 * np.abs(p1).max()
     assert 4 * r1["iterations"] <= r0["iterations"]
     assert rm["iterations"] <= r1["iterations"] + 4
+
+
+@pytest.mark.parametrize("shape", [(30, 26, 35), (40, 36, 44)])
+def test_transposed_hierarchy_matches_oracle_transpose(shape):
+    print(f"case transposed shape={shape}", flush=True)
+    g = configs.build_synthetic_geometry(shape=shape)
+    dg = DeviceGrid(g)
+    A, _ = _system(shape)
+    At = A.T.tocsr()
+    want = _entries(At, shape)
+    got = multigrid.probe_fine_entries(shape, lambda x, out: poisson._matvec(dg, x, out, transpose=True),
+                                       "cuda").cpu().numpy()
+    assert np.max(np.abs(got - want)) <= 1e-12 * np.abs(want).max()
+    mg = poisson.poisson_multigrid(dg, transpose=True)
+    assert mg.transpose and mg is poisson.poisson_multigrid(dg, transpose=True)
+    levels, _ = multigrid.build_levels(torch.from_numpy(want))
+    for a, b in zip(mg.levels64, levels):
+        b = b.numpy()
+        assert np.max(np.abs(a.cpu().numpy() - b)) <= 1e-12 * np.abs(b).max()
+
+
+def test_adjoint_solve_with_transposed_multigrid():
+    shape = (40, 36, 44)
+    print(f"case adjoint shape={shape}", flush=True)
+    g = configs.build_synthetic_geometry(shape=shape)
+    dg = DeviceGrid(g)
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    w = torch.randn(dg.N, generator=gen, dtype=torch.float64, device="cuda")
+    g0, i0 = poisson.poisson_adjoint_device(dg, w, tolerance=1e-12, max_iterations=6000, preconditioner="none")
+    g1, i1 = poisson.poisson_adjoint_device(dg, w, tolerance=1e-12, max_iterations=6000)
+    print("adjoint iterations none", i0["iterations"], "multigrid", i1["iterations"], flush=True)
+    assert i0["converged"] and i1["converged"]
+    assert torch.allclose(g1, g0, rtol=0, atol=1e-9 * float(g0.abs().max()))
+    assert 3 * i1["iterations"] <= i0["iterations"]
 
 
 def test_multigrid_refuses_legacy_grids_and_transpose():
