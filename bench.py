@@ -66,7 +66,10 @@ def parse():
     ap.add_argument("--cpu-quick", action="store_true", help="CPU baseline without the 96^3 Poisson solve")
     ap.add_argument("--no-solvers", action="store_true",
                     help="skip the eigensolve + Poisson timing (FP64 vs FP32 + FP64 refinement)")
-    ap.add_argument("--eig-tol", type=float, default=1e-3, help="residual tolerance of the 96^3 eigensolve")
+    ap.add_argument("--eig-config", choices=("c2", "c5"), default="c2",
+                    help="grid of the synthetic eigensolve + Poisson leg: c2 96^3, k=16 (default, order-2 "
+                         "Poisson); c5 the 224^3, k=48 block of the headline (order-4 Poisson)")
+    ap.add_argument("--eig-tol", type=float, default=1e-3, help="residual tolerance of the synthetic eigensolve")
     ap.add_argument("--eig-block", type=int, default=None,
                     help="working subspace of the 96^3 eigensolve (default: the reference's 12k + 8)")
     ap.add_argument("--molecules", default="benzene,h2o",
@@ -559,7 +562,12 @@ def main():
         geom2, dg2 = cpu_geom, dg
     torch.cuda.empty_cache()
     if not args.no_solvers:
-        line["eigensolve_poisson"] = solver_leg(geom2, dg2, configs.C2, args.eig_tol, order=2, block=args.eig_block)
+        if args.eig_config == "c5":
+            line["eigensolve_poisson_c5"] = solver_leg(cpu_geom, dg, configs.C5_OPERATOR, args.eig_tol, order=4,
+                                                       block=args.eig_block)
+        else:
+            line["eigensolve_poisson"] = solver_leg(geom2, dg2, configs.C2, args.eig_tol, order=2,
+                                                    block=args.eig_block)
         mols = [m for m in args.molecules.split(",") if m and m != "none"]
         for mol in mols:
             torch.cuda.empty_cache()
