@@ -90,8 +90,10 @@ int ig_dots_noalloc_f64(int nd, const double* a0, const double* b0, const double
                         const double* a2, const double* b2, const double* w, long long n, double* out,
                         int o0, int o1, int o2, hipStream_t stream);
 
-/* Fused preconditioned-BiCGSTAB vector updates with device-resident scalars sc[9]
- * (rho, rho_prev, alpha, omega, (rhat,v), (t,s), (t,t), (r,r), breakdown flag);
+/* Fused preconditioned-BiCGSTAB vector updates with device-resident scalars sc[10]
+ * (rho, rho_prev, alpha, omega, (rhat,v), (t,s), (t,t), (r,r), breakdown flag, and the
+ * x step's own breakdown slot, which the x entry folds into the flag; zero sc[9] with the
+ * rest before a solve);
  * dinv = Jacobi preconditioner (NULL = none).
  * Replaces: the SciPy bicgstab solves of poisson/open_boundary.py:1017-1064 (monitor)
  * and :1229-1276 (legacy, Jacobi-preconditioned). */

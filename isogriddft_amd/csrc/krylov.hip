@@ -4,7 +4,8 @@
 // never synchronises with the host:
 //   sc[0] rho = (rhat, r)   sc[1] rho_prev   sc[2] alpha   sc[3] omega
 //   sc[4] (rhat, v)         sc[5] (t, s)     sc[6] (t, t)  sc[7] (r, r)
-//   sc[8] breakdown flag
+//   sc[8] breakdown flag    sc[9] breakdown raised by the last x step (folded into
+//                           sc[8] by the single-block reduction that follows it)
 // The reference solves the same interior systems with SciPy BiCGSTAB
 // (poisson/open_boundary.py:1017-1064, :1229-1276); iterates differ, the discrete
 // solution is the same.
@@ -42,10 +43,11 @@ __global__ __launch_bounds__(IG_THREADS) void dots_partial(const T* __restrict__
 }
 
 // Sum `nparts` partials of ND values; out[slot_i] (op: 0 store, 1 = shift rho).
+// With fold_flag, out[9] (the x step's breakdown) is also folded into the flag out[8].
 template <int ND>
 __global__ __launch_bounds__(IG_THREADS) void dots_final(const double* __restrict__ partial, int nparts,
                                                          double* __restrict__ out, int o0, int o1, int o2,
-                                                         int shift_rho) {
+                                                         int shift_rho, int fold_flag) {
   __shared__ double smem[4 * 3];
   double acc[ND];
 #pragma unroll
@@ -57,6 +59,7 @@ __global__ __launch_bounds__(IG_THREADS) void dots_final(const double* __restric
   ig_block_sum<ND>(acc, smem);
   if (threadIdx.x == 0) {
     if (shift_rho) out[1] = out[0];
+    if (fold_flag && out[9] != 0.0) out[8] = 1.0;
     out[o0] = acc[0];
     if (ND > 1) out[o1] = acc[1];
     if (ND > 2) out[o2] = acc[2];
@@ -80,13 +83,16 @@ int dots_impl(int nd, const T* a0, const T* b0, const T* a1, const T* b1, const 
   const int nb = red_blocks(n);
   if (nd == 1) {
     hipLaunchKernelGGL((dots_partial<T, 1>), dim3(nb), dim3(IG_THREADS), 0, s, a0, b0, a1, b1, a2, b2, w, n, work);
-    hipLaunchKernelGGL((dots_final<1>), dim3(1), dim3(IG_THREADS), 0, s, work, nb, out, o0, o1, o2, shift_rho);
+    hipLaunchKernelGGL((dots_final<1>), dim3(1), dim3(IG_THREADS), 0, s, work, nb, out, o0, o1, o2, shift_rho,
+                       0);
   } else if (nd == 2) {
     hipLaunchKernelGGL((dots_partial<T, 2>), dim3(nb), dim3(IG_THREADS), 0, s, a0, b0, a1, b1, a2, b2, w, n, work);
-    hipLaunchKernelGGL((dots_final<2>), dim3(1), dim3(IG_THREADS), 0, s, work, nb, out, o0, o1, o2, shift_rho);
+    hipLaunchKernelGGL((dots_final<2>), dim3(1), dim3(IG_THREADS), 0, s, work, nb, out, o0, o1, o2, shift_rho,
+                       0);
   } else {
     hipLaunchKernelGGL((dots_partial<T, 3>), dim3(nb), dim3(IG_THREADS), 0, s, a0, b0, a1, b1, a2, b2, w, n, work);
-    hipLaunchKernelGGL((dots_final<3>), dim3(1), dim3(IG_THREADS), 0, s, work, nb, out, o0, o1, o2, shift_rho);
+    hipLaunchKernelGGL((dots_final<3>), dim3(1), dim3(IG_THREADS), 0, s, work, nb, out, o0, o1, o2, shift_rho,
+                       0);
   }
   IG_CHECK_LAUNCH();
   return IG_OK;
@@ -146,7 +152,9 @@ __global__ __launch_bounds__(IG_THREADS) void bicg_x(long long n, T* __restrict_
   // partials re-measure that r, so the host reads the residual of the iterate it gets back.
   // (t, t) = 0 here (t = A shat vanishes, e.g. s = 0 when the rhs spans an invariant
   // subspace) still takes x += alpha phat, r = s, then raises the flag: the next steps,
-  // whose beta would divide by omega = 0, are no-ops.  Every block sees the same scalars.
+  // whose beta would divide by omega = 0, are no-ops.  Every block sees the same scalars:
+  // this kernel raises sc[9], never sc[8], because a block that starts after block 0 has
+  // finished would otherwise read the new flag and skip the update the others made.
   const bool frozen = sc[8] != 0.0;
   const bool broken = frozen || !(sc[6] > 0.0);
   const double alpha = sc[2];
@@ -165,9 +173,9 @@ __global__ __launch_bounds__(IG_THREADS) void bicg_x(long long n, T* __restrict_
     acc[0] += (double)rhat[t] * (double)rn;
     acc[1] += (double)rn * (double)rn;
   }
-  ig_block_sum<2>(acc, smem);  // its barriers order every lane's read of sc[8] before the write
+  ig_block_sum<2>(acc, smem);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    if (broken) sc[8] = 1.0;
+    if (broken) sc[9] = 1.0;
     else sc[3] = omega;
   }
   if (threadIdx.x == 0) {
@@ -200,7 +208,7 @@ int bicg_step_x(long long n, T* x, const T* phat, const T* shat, const T* sv, co
   if (n <= 0 || !x || !phat || !shat || !sv || !tv || !r || !rhat || !sc || !work) return IG_EINVAL;
   const int nb = red_blocks(n);
   hipLaunchKernelGGL(bicg_x<T>, dim3(nb), dim3(IG_THREADS), 0, s, n, x, phat, shat, sv, tv, r, rhat, sc, work);
-  hipLaunchKernelGGL((dots_final<2>), dim3(1), dim3(IG_THREADS), 0, s, work, nb, sc, 0, 7, 0, 1);
+  hipLaunchKernelGGL((dots_final<2>), dim3(1), dim3(IG_THREADS), 0, s, work, nb, sc, 0, 7, 0, 1, 1);
   IG_CHECK_LAUNCH();
   return IG_OK;
 }

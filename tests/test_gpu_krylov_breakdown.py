@@ -64,6 +64,9 @@ def test_x_step_with_vanishing_t_takes_the_update_then_flags():
     _call("ig_bicgstab_x_f64", N, P(b["x"]), P(b["phat"]), P(b["shat"]), P(b["s"]), P(b["t"]), P(b["r"]),
           P(b["rhat"]), P(sc), P(work))
     torch.cuda.synchronize()
+    # the x kernel raises its own slot (no block of it may see a flag it raised itself);
+    # the reduction launched after it folds that slot into the flag
+    assert sc[9].item() == 1.0
     assert sc[8].item() == 1.0
     xr = keep["x"] + alpha * keep["phat"]
     assert float((b["x"] - xr).abs().max() / xr.abs().max()) <= 1e-15
