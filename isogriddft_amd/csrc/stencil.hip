@@ -39,6 +39,8 @@ struct MonParams {
   T scale;
   int interior_only;
   int nty, ntz, nxc, xc, units;
+  int order;  // block order of the full-z kernel: 0 units of one XCD consecutive, 1 y sweep,
+              // 2 y sweep in bands of eight pairs
 };
 
 // Coefficient (times 2h) of row `row` at column offset `off` of the 1D derivative.
@@ -1425,7 +1427,12 @@ int launch_monitor_zq_t(const MonParams<T>& base, hipStream_t stream, bool auto_
 // step that does not depend on the orbital (6 + 2 coefficient loads per lane, the
 // barrier) is shared by the NB orbitals of the workgroup, so NB = 3 is used: 12
 // wavefronts at <= 168 VGPRs (3 per SIMD) and 144 KB of LDS fill a CU.  Same expression
-// tree as the general kernel: bit-identical results.
+// tree as the general kernel: bit-identical results.  Block order (MonParams::order): the
+// y-halo rows of a tile are the interior rows of its y neighbours, and the halo planes of
+// an x chunk are the last planes of the previous chunk; ordering the workgroups so that
+// y neighbours run at the same time (across XCDs, through the memory-side cache) and an
+// XCD takes the chunks of one tile pair in sequence (through its L2) serves most of both
+// from cache instead of HBM.
 template <typename T>
 __device__ __forceinline__ T lane_from(int addr, T v);
 template <>
@@ -1459,11 +1466,46 @@ __global__ __launch_bounds__((TY + 4) * 64) void monitor_zp_kernel(MonParams<T> 
   const int xcd = b & 7;
   const int r = b >> 3;
   const int grp = r % ngroups;
-  const int upx = (p.units + 7) / 8;
-  const int u = xcd * upx + r / ngroups;
-  if (u >= p.units) return;
-  const int tyi = u % p.nty;
-  const int xci = u / p.nty;
+  int tyi, xci;
+  if (p.order == 0) {
+    // each XCD takes a contiguous range of units (y tiles fastest), the orbital groups of
+    // a unit back to back: two y-adjacent units share their boundary rows in L2
+    const int upx = (p.units + 7) / 8;
+    const int u = xcd * upx + r / ngroups;
+    if (u >= p.units) return;
+    tyi = u % p.nty;
+    xci = u / p.nty;
+  } else {
+    // y sweep: an item is a pair of y-adjacent tiles at one x chunk; at a given moment the
+    // eight XCDs hold eight consecutive pairs of the same chunk, so the rows on either side
+    // of an XCD boundary are fetched by two XCDs at about the same time (one HBM read,
+    // the other served by the memory-side cache)
+    const int v = r / ngroups;
+    const int npairs = (p.nty + 1) >> 1;
+    const int item = (v >> 1) * 8 + xcd;
+    if (item >= npairs * p.nxc) return;
+    int pair;
+    if (p.order == 1) {
+      xci = item / npairs;
+      pair = item % npairs;
+    } else {
+      // bands of eight pairs swept chunk by chunk: XCD c keeps pair c of a full band for
+      // all its chunks, so the halo planes of a chunk were just read by the same XCD
+      const int full = (npairs >> 3) * 8 * p.nxc;
+      if (item < full) {
+        const int w = item % (8 * p.nxc);
+        xci = w >> 3;
+        pair = (item / (8 * p.nxc)) * 8 + (w & 7);
+      } else {
+        const int rp = npairs & 7;
+        const int w = item - full;
+        xci = w / rp;
+        pair = (npairs & ~7) + w % rp;
+      }
+    }
+    tyi = 2 * pair + (v & 1);
+    if (tyi >= p.nty) return;
+  }
   const int nx = p.nx, ny = p.ny, nz = p.nz;
   const unsigned plane_bytes = (unsigned)(ny * nz) * S;
   const int j0 = max(min(tyi * TY, ny - TY), 0);
@@ -1665,7 +1707,8 @@ int launch_monitor_zp_t(const MonParams<T>& base, hipStream_t stream, bool auto_
   if (p.xc > p.nx) p.xc = p.nx;
   p.nxc = (p.nx + p.xc - 1) / p.xc;
   p.units = p.nty * p.nxc;
-  const long long blocks = 8LL * groups * ((p.units + 7) / 8);
+  const long long items = (long long)((p.nty + 1) / 2) * p.nxc;
+  const long long blocks = p.order == 0 ? 8LL * groups * ((p.units + 7) / 8) : 16LL * groups * ((items + 7) / 8);
   if (blocks <= 0 || blocks > 0x7fffffffLL) return IG_EINVAL;
   if (auto_pick && blocks < 512) return IG_EUNSUPPORTED;
   hipLaunchKernelGGL((monitor_zp_kernel<T, NB, GD, TY>), dim3((unsigned)blocks), dim3((TY + 4) * 64), 0, stream, p);
@@ -1690,10 +1733,17 @@ int launch_monitor(const MonParams<T>& p, int closure, int transpose, int tile, 
   if (tile >= 9200000) {
     if (closure) return IG_EUNSUPPORTED;
     switch (tile - 9200000) {
-      // rows per tile x 10 + orbitals per workgroup (+ 100: coefficients two planes ahead)
+      // rows per tile x 10 + orbitals per workgroup (+ 100: coefficients two planes ahead,
+      // + 1000: y-sweep block order)
       case 81: return launch_monitor_zp_t<T, 1, 1, 8>(p, s);
       case 82: return launch_monitor_zp_t<T, 2, 1, 8>(p, s);
       case 83: return launch_monitor_zp_t<T, 3, 1, 8>(p, s);
+      case 1083:
+      case 2083: {  // the same with the y-sweep block orders
+        MonParams<T> q = p;
+        q.order = (tile - 9200000) / 1000;
+        return launch_monitor_zp_t<T, 3, 1, 8>(q, s);
+      }
       case 182: return launch_monitor_zp_t<T, 2, 2, 8>(p, s);
       case 122: return launch_monitor_zp_t<T, 2, 1, 12>(p, s);
       case 43: return launch_monitor_zp_t<T, 3, 1, 4>(p, s);
@@ -1732,13 +1782,18 @@ int launch_monitor(const MonParams<T>& p, int closure, int transpose, int tile, 
   // 1.69 ms against 1.90 ms for the general kernel, FP64 3.22 ms against 3.70 ms for the
   // z-vectorised one).
   // Large blocks (geometry beyond the Infinity Cache): FP32 rows that fit one wavefront
-  // take the pipelined full-z kernel (224^3, k = 48: 1.24 ms against 1.67 ms for the
-  // register-queue kernel), FP64 the z-vectorised register-queue kernel (3.03 ms against
+  // take the pipelined full-z kernel in the banded y-sweep order (224^3, k = 48: 1.15 ms;
+  // 1.24 ms in unit order against 1.67 ms for the register-queue kernel), FP64 the
+  // z-vectorised register-queue kernel (3.03 ms against
   // 3.22 ms); shapes these decline fall back to the register-queue kernel.
   if (tile == 0 && closure == 0 && !transpose && !p.interior_only &&
       (long long)p.nx * p.ny * p.nz * (long long)sizeof(T) * 8 > (128LL << 20)) {
     int st = IG_EUNSUPPORTED;
-    if (sizeof(T) == 4 && p.nblk >= 3 && p.nz >= 128) st = launch_monitor_zp_t<T, 3, 1, 8>(p, s, true);
+    if (sizeof(T) == 4 && p.nblk >= 3 && p.nz >= 128) {
+      MonParams<T> q = p;
+      q.order = 2;  // banded y sweep: 224^3, k = 48: 1.150 ms against 1.215 ms in unit order
+      st = launch_monitor_zp_t<T, 3, 1, 8>(q, s, true);
+    }
     if (st == IG_EUNSUPPORTED && sizeof(T) == 8 && p.nblk >= 3) st = launch_monitor_zq_t<T, 3>(p, s, true);
     if (st == IG_EUNSUPPORTED && sizeof(T) == 4 && p.nblk >= 6) st = launch_monitor_rq_t<T, 6>(p, s, true);
     else if (st == IG_EUNSUPPORTED && sizeof(T) == 8 && p.nblk >= 4) st = launch_monitor_rq_t<T, 4>(p, s, true);

@@ -23,7 +23,7 @@ pytestmark = pytest.mark.gpu
 
 ZQ = 9000000  # tile code forcing the z-vectorised register-queue kernel (+ orbitals per workgroup)
 ZP = 9200000  # tile code forcing the pipelined full-z kernel (+ rows x 10 + orbitals, + 100: deeper prefetch)
-ZP_VARIANTS = (81, 82, 83, 182, 122, 43, 64)
+ZP_VARIANTS = (81, 82, 83, 182, 122, 43, 64, 1083, 2083)  # + 1000 / 2000: y-sweep block orders
 GENERAL = 23208  # the general kernel: 2 orbitals, 8 x 32 tiles
 
 
@@ -119,7 +119,8 @@ def test_zp_matches_oracle_fp64_and_fp32(shape):
             assert err <= tol, (shape, dt, var, err)
 
 
-@pytest.mark.parametrize("shape", [(33, 37, 72), (130, 24, 40), (9, 61, 100), (12, 30, 256)])
+# (20, 256, 32): 32 tiles of 8 rows, two full bands of eight tile pairs in the y-sweep orders
+@pytest.mark.parametrize("shape", [(33, 37, 72), (130, 24, 40), (9, 61, 100), (12, 30, 256), (20, 256, 32)])
 def test_zp_bitwise_equal_to_general_kernel(shape):
     print(f"case shape={shape}", flush=True)
     g = configs.build_synthetic_geometry(shape=shape)
