@@ -1732,22 +1732,19 @@ int launch_monitor(const MonParams<T>& p, int closure, int transpose, int tile, 
   if (tile < 0) return IG_EINVAL;
   if (tile >= 9200000) {
     if (closure) return IG_EUNSUPPORTED;
-    switch (tile - 9200000) {
-      // rows per tile x 10 + orbitals per workgroup (+ 100: coefficients two planes ahead,
-      // + 1000: y-sweep block order)
-      case 81: return launch_monitor_zp_t<T, 1, 1, 8>(p, s);
-      case 82: return launch_monitor_zp_t<T, 2, 1, 8>(p, s);
-      case 83: return launch_monitor_zp_t<T, 3, 1, 8>(p, s);
-      case 1083:
-      case 2083: {  // the same with the y-sweep block orders
-        MonParams<T> q = p;
-        q.order = (tile - 9200000) / 1000;
-        return launch_monitor_zp_t<T, 3, 1, 8>(q, s);
-      }
-      case 182: return launch_monitor_zp_t<T, 2, 2, 8>(p, s);
-      case 122: return launch_monitor_zp_t<T, 2, 1, 12>(p, s);
-      case 43: return launch_monitor_zp_t<T, 3, 1, 4>(p, s);
-      case 64: return launch_monitor_zp_t<T, 4, 1, 6>(p, s);
+    // rows per tile x 10 + orbitals per workgroup (+ 100: coefficients two planes ahead),
+    // + 1000 x block order (0 unit order, 1 y sweep, 2 banded y sweep)
+    MonParams<T> q = p;
+    q.order = (tile - 9200000) / 1000;
+    if (q.order > 2) return IG_EUNSUPPORTED;
+    switch ((tile - 9200000) % 1000) {
+      case 81: return launch_monitor_zp_t<T, 1, 1, 8>(q, s);
+      case 82: return launch_monitor_zp_t<T, 2, 1, 8>(q, s);
+      case 83: return launch_monitor_zp_t<T, 3, 1, 8>(q, s);
+      case 182: return launch_monitor_zp_t<T, 2, 2, 8>(q, s);
+      case 122: return launch_monitor_zp_t<T, 2, 1, 12>(q, s);
+      case 43: return launch_monitor_zp_t<T, 3, 1, 4>(q, s);
+      case 64: return launch_monitor_zp_t<T, 4, 1, 6>(q, s);
       default: return IG_EUNSUPPORTED;
     }
   }
