@@ -41,6 +41,8 @@ struct MonParams {
   int nty, ntz, nxc, xc, units;
   int order;  // block order of the full-z kernel: 0 units of one XCD consecutive, 1 y sweep,
               // 2 y sweep in bands of eight pairs
+  int ug;     // block order of the 32-point-tile kernel: 0 units of one XCD consecutive,
+              // g > 0 items of g consecutive units dealt round robin to the XCDs
 };
 
 // Coefficient (times 2h) of row `row` at column offset `off` of the 1D derivative.
@@ -1188,8 +1190,17 @@ __global__ __launch_bounds__(IG_THREADS, 2) void monitor_zq_kernel(MonParams<T> 
   const int xcd = b & 7;
   const int r = b >> 3;
   const int grp = r % ngroups;
-  const int upx = (p.units + 7) / 8;
-  const int u = xcd * upx + r / ngroups;
+  int u;
+  if (p.ug <= 0) {
+    const int upx = (p.units + 7) / 8;
+    u = xcd * upx + r / ngroups;
+  } else {
+    // items of ug consecutive units (z tiles fastest) dealt round robin to the XCDs: at a
+    // given moment the eight XCDs hold 8 ug consecutive units, so the z- and y-halo lines
+    // at the edges of an XCD's units are fetched by two XCDs at about the same time
+    const int v = r / ngroups;
+    u = ((v / p.ug) * 8 + xcd) * p.ug + v % p.ug;
+  }
   if (u >= p.units) return;
   const int tzi = u % p.ntz;
   const int tyi = (u / p.ntz) % p.nty;
@@ -1400,7 +1411,8 @@ int launch_monitor_zq_t(const MonParams<T>& base, hipStream_t stream, bool auto_
   if (p.xc > p.nx) p.xc = p.nx;
   p.nxc = (p.nx + p.xc - 1) / p.xc;
   p.units = p.nty * p.ntz * p.nxc;
-  const long long blocks = 8LL * groups * ((p.units + 7) / 8);
+  const long long per = p.ug > 0 ? 8LL * p.ug : 8LL;
+  const long long blocks = 8LL * groups * ((p.units + per - 1) / per) * (p.ug > 0 ? p.ug : 1);
   if (blocks <= 0 || blocks > 0x7fffffffLL) return IG_EINVAL;
   if (auto_pick && blocks < 2048) return IG_EUNSUPPORTED;
   hipLaunchKernelGGL((monitor_zq_kernel<T, NB>), dim3((unsigned)blocks), dim3(IG_THREADS), 0, stream, p);
@@ -1750,11 +1762,15 @@ int launch_monitor(const MonParams<T>& p, int closure, int transpose, int tile, 
   }
   if (tile >= 9000000) {
     if (closure) return IG_EUNSUPPORTED;
-    switch (tile - 9000000) {
-      case 1: return launch_monitor_zq_t<T, 1>(p, s);
-      case 2: return launch_monitor_zq_t<T, 2>(p, s);
-      case 3: return launch_monitor_zq_t<T, 3>(p, s);
-      case 4: return launch_monitor_zq_t<T, 4>(p, s);
+    // orbitals per workgroup, + 100 x units per interleaved item (0: contiguous ranges)
+    MonParams<T> q = p;
+    q.ug = (tile - 9000000) / 100;
+    if (q.ug > 64) return IG_EUNSUPPORTED;
+    switch ((tile - 9000000) % 100) {
+      case 1: return launch_monitor_zq_t<T, 1>(q, s);
+      case 2: return launch_monitor_zq_t<T, 2>(q, s);
+      case 3: return launch_monitor_zq_t<T, 3>(q, s);
+      case 4: return launch_monitor_zq_t<T, 4>(q, s);
       default: return IG_EUNSUPPORTED;
     }
   }
@@ -1781,8 +1797,9 @@ int launch_monitor(const MonParams<T>& p, int closure, int transpose, int tile, 
   // Large blocks (geometry beyond the Infinity Cache): FP32 rows that fit one wavefront
   // take the pipelined full-z kernel in the banded y-sweep order (224^3, k = 48: 1.15 ms;
   // 1.24 ms in unit order against 1.67 ms for the register-queue kernel), FP64 the
-  // z-vectorised register-queue kernel (3.03 ms against
-  // 3.22 ms); shapes these decline fall back to the register-queue kernel.
+  // z-vectorised register-queue kernel with unit pairs dealt round robin to the XCDs
+  // (2.74 ms; 3.03 ms in contiguous ranges against 3.22 ms); shapes these decline fall back
+  // to the register-queue kernel.
   if (tile == 0 && closure == 0 && !transpose && !p.interior_only &&
       (long long)p.nx * p.ny * p.nz * (long long)sizeof(T) * 8 > (128LL << 20)) {
     int st = IG_EUNSUPPORTED;
@@ -1791,7 +1808,11 @@ int launch_monitor(const MonParams<T>& p, int closure, int transpose, int tile, 
       q.order = 2;  // banded y sweep: 224^3, k = 48: 1.150 ms against 1.215 ms in unit order
       st = launch_monitor_zp_t<T, 3, 1, 8>(q, s, true);
     }
-    if (st == IG_EUNSUPPORTED && sizeof(T) == 8 && p.nblk >= 3) st = launch_monitor_zq_t<T, 3>(p, s, true);
+    if (st == IG_EUNSUPPORTED && sizeof(T) == 8 && p.nblk >= 3) {
+      MonParams<T> q = p;
+      q.ug = 2;  // pairs of units dealt round robin: 224^3, k = 48: 2.744 ms against 3.037 ms
+      st = launch_monitor_zq_t<T, 3>(q, s, true);
+    }
     if (st == IG_EUNSUPPORTED && sizeof(T) == 4 && p.nblk >= 6) st = launch_monitor_rq_t<T, 6>(p, s, true);
     else if (st == IG_EUNSUPPORTED && sizeof(T) == 8 && p.nblk >= 4) st = launch_monitor_rq_t<T, 4>(p, s, true);
     if (st != IG_EUNSUPPORTED) return st;

@@ -70,7 +70,8 @@ def test_zq_bitwise_equal_to_general_kernel(shape):
             x = torch.from_numpy(_fields(shape, k, 9)).to("cuda", dt)
             vt = torch.from_numpy(0.2 * np.cos(g.y_points)).to("cuda", dt)
             gen = kinetic.apply_operator(dg, x, scale=-0.5, closure="trial_fix", veff=vt, tile=GENERAL)
-            for nb in (1, 2, 3, 4):
+            # + 100 g: items of g consecutive units dealt round robin to the XCDs
+            for nb in (1, 2, 3, 4, 103, 203, 303, 802):
                 for xchunk in (0, 9, shape[0]):
                     out = kinetic.apply_operator(dg, x, scale=-0.5, closure="trial_fix", veff=vt, tile=ZQ + nb,
                                                  xchunk=xchunk)
