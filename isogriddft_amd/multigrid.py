@@ -19,7 +19,8 @@ coarse spaces respect that split:
                 D^-1 A from power iterations, the coarse one COARSE_OMEGA
 
 The fine same-parity entries (diagonal and +-2 e_a couplings, closure included) are read
-off the operator kernel itself by applying it to 28 probe vectors (``probe_keys``).  The hierarchy depends on the geometry only and is built once per
+off the operator kernel itself by applying it to 28 probe vectors
+(``probe_fine_entries``).  The hierarchy depends on the geometry only and is built once per
 grid.  With it the benzene 224^3 monitor system needs a few tens of BiCGSTAB iterations
 instead of ~280, and FP32 converges as fast as FP64 down to its rounding floor.  The
 preconditioner changes the iterates, not the discrete solution the solve converges to.
@@ -240,11 +241,16 @@ class PoissonMultigrid:
         return self.buffers(dtype)["dw"]
 
     def finish(self, b, x, stream=None):
-        """Complete the V-cycle x ~ A^-1 b from its first fine sweep x = dw * b (in place)."""
+        """Complete the V-cycle x ~ A^-1 b from its first fine sweep x = dw * b (in place).
+        With a stream, every launch of the cycle (operator applies, kernels, the coarsest
+        products) goes to that stream; otherwise to the current one."""
+        if stream is not None:
+            with torch.cuda.stream(stream):
+                return self.finish(b, x)
         dt = x.dtype
         B = self.buffers(dt)
         sfx = "f64" if dt == torch.float64 else "f32"
-        h = _lib.stream_handle(stream)
+        h = _lib.stream_handle()
         nx, ny, nz = self.shape
         cx, cy, cz = B["dims"][0]
         t = B["t"]
@@ -257,9 +263,12 @@ class PoissonMultigrid:
         _lib.call(f"ig_mg_smooth_{sfx}", _lib.ptr(x), _lib.ptr(b), _lib.ptr(t), _lib.ptr(B["dw"]), self.n, h)
 
     def apply(self, b, out, stream=None):
-        """out = M b (one V-cycle from zero)."""
+        """out = M b (one V-cycle from zero), on ``stream`` if given."""
+        if stream is not None:
+            with torch.cuda.stream(stream):
+                return self.apply(b, out)
         torch.mul(b, self.scaled_inverse_diagonal(b.dtype), out=out)
-        self.finish(b, out, stream)
+        self.finish(b, out)
 
     def _coarse(self, B, lev, sfx, h):
         """V-cycle on level lev + 1 (coarse index lev) for the right-hand side in
