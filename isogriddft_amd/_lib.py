@@ -8,6 +8,8 @@ for the HIP kernels.
 from __future__ import annotations
 
 import ctypes as C
+import functools
+import inspect
 from pathlib import Path
 
 import torch
@@ -157,3 +159,29 @@ def ptr(t):
 def stream_handle(stream=None):
     s = torch.cuda.current_stream() if stream is None else stream
     return C.c_void_p(s.cuda_stream)
+
+
+def on_stream(fn):
+    """Decorator for functions with a ``stream`` argument (a torch CUDA stream or None):
+    with a stream, the body runs with that stream current, so its tensor operations and
+    the library kernels it launches (on the current stream when ``stream`` is None) are
+    ordered on that one stream.  Without one, everything runs on the current stream.  The
+    caller orders later uses of the results, as with any stream argument."""
+    idx = list(inspect.signature(fn).parameters).index("stream")
+
+    @functools.wraps(fn)
+    def wrapped(*args, **kw):
+        if len(args) > idx:
+            stream = args[idx]
+            if stream is None:
+                return fn(*args, **kw)
+            args = args[:idx] + (None,) + args[idx + 1:]
+        else:
+            stream = kw.get("stream")
+            if stream is None:
+                return fn(*args, **kw)
+            kw["stream"] = None
+        with torch.cuda.stream(stream):
+            return fn(*args, **kw)
+
+    return wrapped

@@ -61,6 +61,7 @@ def evaluate_ion_ion_repulsion(case=H2_BENCHMARK_CASE) -> float:
     return e
 
 
+@_lib.on_stream
 def density_device(dg: DeviceGrid, orbitals: torch.Tensor, occupations, out=None, accumulate=False, stream=None):
     """rho = sum_m occ_m psi_m^2 (FP64, flat) for an orbital-major block."""
     if orbitals.dtype not in (torch.float32, torch.float64):

@@ -22,7 +22,7 @@ from dataclasses import dataclass
 import numpy as np
 import torch
 
-from . import configs
+from . import _lib, configs
 from .device import DeviceGrid, device_grid
 from .kinetic import apply_operator
 
@@ -58,6 +58,7 @@ class BlockHamiltonian:
         self.veff[torch.float32] = self.veff[torch.float64].float()
         self.nonlocal_plan = nonlocal_plan if (nonlocal_plan is not None and not nonlocal_plan.empty) else None
 
+    @_lib.on_stream
     def apply(self, X: torch.Tensor, out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
         """out = H X for a block X (k, *shape) or one field (*shape)."""
         v = self.veff[X.dtype].view(self.dg.shape)

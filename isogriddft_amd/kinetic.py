@@ -33,6 +33,7 @@ def _closure_id(closure: str) -> int:
         raise ValueError("closure must be `production` or `trial_fix`; received " f"`{closure}`.") from None
 
 
+@_lib.on_stream
 def apply_operator(
     dg: DeviceGrid,
     x: torch.Tensor,
@@ -198,6 +199,7 @@ def _field_tensor(field, dg):
     return t.contiguous()
 
 
+@_lib.on_stream
 def green_identity_terms(field, grid_geometry, closure: str = "trial_fix", *, stream=None) -> GreenIdentityTerms:
     """Operator energy <f, T f>_W with T of ``closure``, the gradient-form energy
     1/2 sum w d_a f g^{ab} d_b f (np.gradient edge_order=2), the explicit boundary-flux
@@ -234,6 +236,7 @@ def green_identity_terms(field, grid_geometry, closure: str = "trial_fix", *, st
     )
 
 
+@_lib.on_stream
 def green_identity_defect(u, v, grid_geometry, closure: str = "production", *, stream=None) -> float:
     """<u, T v>_W - <T u, v>_W: the mismatch between the kinetic operator and its adjoint
     under the quadrature weights (zero to round-off for the zero-ghost closure, whose

@@ -240,13 +240,10 @@ class PoissonMultigrid:
         """omega D^-1 (flat, zero on the faces): x = dw * b is the V-cycle's first sweep."""
         return self.buffers(dtype)["dw"]
 
+    @_lib.on_stream
     def finish(self, b, x, stream=None):
-        """Complete the V-cycle x ~ A^-1 b from its first fine sweep x = dw * b (in place).
-        With a stream, every launch of the cycle (operator applies, kernels, the coarsest
-        products) goes to that stream; otherwise to the current one."""
-        if stream is not None:
-            with torch.cuda.stream(stream):
-                return self.finish(b, x)
+        """Complete the V-cycle x ~ A^-1 b from its first fine sweep x = dw * b (in place),
+        every launch on ``stream`` if given, else on the current stream."""
         dt = x.dtype
         B = self.buffers(dt)
         sfx = "f64" if dt == torch.float64 else "f32"
@@ -262,11 +259,9 @@ class PoissonMultigrid:
         self.matvec(x, t)
         _lib.call(f"ig_mg_smooth_{sfx}", _lib.ptr(x), _lib.ptr(b), _lib.ptr(t), _lib.ptr(B["dw"]), self.n, h)
 
+    @_lib.on_stream
     def apply(self, b, out, stream=None):
         """out = M b (one V-cycle from zero), on ``stream`` if given."""
-        if stream is not None:
-            with torch.cuda.stream(stream):
-                return self.apply(b, out)
         torch.mul(b, self.scaled_inverse_diagonal(b.dtype), out=out)
         self.finish(b, out)
 

@@ -31,8 +31,6 @@ the residual is inspected every ``check_every`` iterations.  Multipole orders 0-
 
 from __future__ import annotations
 
-import functools
-import inspect
 import itertools
 import math
 from dataclasses import dataclass
@@ -358,28 +356,7 @@ def _split_correction(dg, pdata, rho, order, stream=None):
     return boundary_field(dg, mom.cpu().numpy(), order, center, stream), center
 
 
-def _on_stream(fn):
-    """Run ``fn`` with its ``stream`` argument (a torch CUDA stream) made current, so that
-    the tensor operations inside it and the library kernels (launched on the current stream
-    when no stream is given) are ordered on that one stream; without a stream, everything
-    runs on the current stream.  The caller orders later uses of the results, as with any
-    stream argument."""
-    sig = inspect.signature(fn)
-
-    @functools.wraps(fn)
-    def wrapped(*args, **kw):
-        bound = sig.bind(*args, **kw)
-        stream = bound.arguments.get("stream")
-        if stream is None:
-            return fn(*args, **kw)
-        bound.arguments["stream"] = None
-        with torch.cuda.stream(stream):
-            return fn(*bound.args, **bound.kwargs)
-
-    return wrapped
-
-
-@_on_stream
+@_lib.on_stream
 def boundary_field(dg, flat, order, center=None, stream=None):
     """Multipole potential of the flat raw moments `flat` (about `center`, default the grid
     reference centre) on the six faces, zero inside (open_boundary.py:467-495)."""
@@ -412,7 +389,7 @@ def _device_rho(dg, rho):
     return rho.reshape(-1).contiguous()
 
 
-@_on_stream
+@_lib.on_stream
 def multipole_boundary_device(dg, rho, order=2, mode="corrected_moments", stream=None):
     """Boundary field b (zero inside) and the flat raw moments used, on the device.
 
@@ -478,7 +455,7 @@ def _matvec(dg, x, out, stream=None, transpose=False):
     out.mul_(poisson_data(dg).interior_mask(out.dtype))
 
 
-@_on_stream
+@_lib.on_stream
 def bicgstab(dg, b, x, *, dinv=None, rtol=1e-8, maxiter=400, check_every=8, stream=None, transpose=False,
              mg=None):
     """Solve (-L) x = b on interior nodes (transpose: (-L)^T x = b); x (flat CUDA tensor)
@@ -607,7 +584,7 @@ def poisson_multigrid(dg, transpose=False):
     return mg
 
 
-@_on_stream
+@_lib.on_stream
 def solve_poisson_device(dg, rho, *, multipole_order=2, tolerance=1e-8, max_iterations=400, solver="auto",
                          mode="corrected_moments", precision="fp64", check_every=8, inner_tolerance=None,
                          stream=None, preconditioner="auto"):
@@ -710,7 +687,7 @@ def solve_poisson_device(dg, rho, *, multipole_order=2, tolerance=1e-8, max_iter
 # ------------------------------------------------------------------ adjoint
 
 
-@_on_stream
+@_lib.on_stream
 def moments_adjoint_device(dg, q, order, mode="corrected_moments", out=None, stream=None):
     """sum_j q_j d moment_j / d rho (flat FP64 field) for the flat raw moments that
     multipole_boundary_device reports (nodal; on monitor grids plus the corrected-moment
@@ -737,7 +714,7 @@ def moments_adjoint_device(dg, q, order, mode="corrected_moments", out=None, str
     return out
 
 
-@_on_stream
+@_lib.on_stream
 def poisson_adjoint_device(dg, g, multipole_order=2, mode="corrected_moments", tolerance=1e-10,
                            max_iterations=4000, stream=None, preconditioner="auto"):
     """P^T g for the open-boundary Poisson map v = P rho of solve_poisson_device: the

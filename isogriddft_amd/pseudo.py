@@ -93,6 +93,7 @@ def _atom_rows(atoms):
     return rows
 
 
+@_lib.on_stream
 def local_potential_device(dg: DeviceGrid, atoms, out=None, accumulate=False, stream=None):
     """FP64 V_loc summed over atoms on the device (flat, length N)."""
     if out is None:
@@ -108,6 +109,7 @@ def local_potential_device(dg: DeviceGrid, atoms, out=None, accumulate=False, st
     return out
 
 
+@_lib.on_stream
 def local_potential_gradient_device(dg: DeviceGrid, atoms, rho_w, stream=None):
     """dE/dR_A (natoms x 3, host) of E = sum rho w V_loc; rho_w = rho * w on the device."""
     if len(atoms) > 64:
@@ -312,6 +314,7 @@ class _AtomPatch:
                   nbytes, _lib.ptr(self.idx), _lib.ptr(self.dist), _lib.stream_handle())
         del work
 
+    @_lib.on_stream
     def quadrature(self, dg, rho, values=None, stream=None):
         """(sum over patch points of rho_interp V_A) * patch cell volume, on the device."""
         out = torch.zeros(1, dtype=torch.float64, device=dg.device)
@@ -323,6 +326,7 @@ class _AtomPatch:
                   _lib.ptr(self.vq), _lib.ptr(values), _lib.ptr(work), _lib.ptr(out), _lib.stream_handle(stream))
         return out * self.cell_volume
 
+    @_lib.on_stream
     def main_grid(self, dg, rho, stream=None):
         """[sum rho V_A w, sum rho w, node count] over the main-grid nodes inside the sphere."""
         lib = _lib.load()
@@ -334,6 +338,7 @@ class _AtomPatch:
         del keep
         return out
 
+    @_lib.on_stream
     def add_field(self, dg, out, scale, constant, stream=None):
         aptr, keep = _host_atom(self.row)
         _lib.call("ig_patch_field_f64", _lib.ptr(dg.xyz()), dg.N, aptr, self.radius, int(constant), float(scale),
@@ -354,6 +359,7 @@ class PatchPlan:
             for n, (el, pos) in enumerate(atoms)
         ]
 
+    @_lib.on_stream
     def corrections(self, dg, rho, stream=None):
         """Per atom (pseudo/local.py:461-517): main-grid and patch energies, correction
         energy, density weight and node count inside the sphere (host floats)."""
@@ -367,6 +373,7 @@ class PatchPlan:
             res.append((p, main_e, patch_e, ce, dw, int(round(cnt))))
         return res
 
+    @_lib.on_stream
     def embed(self, dg, corrections, out=None, stream=None):
         """Frozen correction field (pseudo/local.py:546-582) accumulated into `out`;
         returns (field, per-atom scaling factors)."""
@@ -407,6 +414,7 @@ def patch_plan(dg: DeviceGrid, case, grid_geometry, params: LocalPotentialPatchP
     return plan
 
 
+@_lib.on_stream
 def frozen_patch_field_device(dg: DeviceGrid, case, grid_geometry, rho, params=None, stream=None):
     """Device-level frozen patch embedding for a flat FP64 density: (correction field,
     per-atom correction records, scaling factors)."""
@@ -574,6 +582,7 @@ class NonlocalPlan:
     def empty(self):
         return not self.per_atom
 
+    @_lib.on_stream
     def project(self, psi, stream=None):
         """[(atom, coef (k, nproj), d = coef H^T (k, nproj))] for a block psi (k, *shape)."""
         dg = self.dg
@@ -597,6 +606,7 @@ class NonlocalPlan:
             out.append((ap, coef, d))
         return out
 
+    @_lib.on_stream
     def apply(self, psi, out, stream=None, projections=None):
         """out += V_nl psi for a block (out, psi of one dtype and shape).  Returns the projections."""
         dg = self.dg
@@ -617,6 +627,7 @@ class NonlocalPlan:
                       ap.nproj, _lib.ptr(d), _lib.stream_handle(stream))
         return projections
 
+    @_lib.on_stream
     def energy_terms(self, psi, stream=None):
         """<psi_m | V_nl | psi_m>_W per orbital (host, FP64)."""
         dg = self.dg

@@ -24,6 +24,7 @@ def workspace(device, n_doubles: int, slot: str = "red") -> torch.Tensor:
     return t
 
 
+@_lib.on_stream
 def dots_into(out: torch.Tensor, pairs, w=None, slots=(0, 1, 2), stream=None):
     """out[slots[i]] = sum(a_i * b_i [* w]) for up to three pairs (device-resident)."""
     nd = len(pairs)
@@ -118,6 +119,7 @@ def _alpha(a, k, name):
     return a
 
 
+@_lib.on_stream
 def row_dots(a: torch.Tensor, b: torch.Tensor, w: torch.Tensor | None = None, stream=None) -> torch.Tensor:
     """FP64 device vector out[i] = sum_n w[n] a[i, n] b[i, n] (w None: 1)."""
     a, b = _rows(a), _rows(b)
@@ -134,6 +136,7 @@ def row_dots(a: torch.Tensor, b: torch.Tensor, w: torch.Tensor | None = None, st
     return out
 
 
+@_lib.on_stream
 def cg_update(x, r, p, q, dinv, w, alpha, stream=None) -> torch.Tensor:
     """Row i: x += alpha_i p, r -= alpha_i q; returns the FP64 vector sum w dinv r_i^2."""
     for t in (x, r, p, q):
@@ -152,6 +155,7 @@ def cg_update(x, r, p, q, dinv, w, alpha, stream=None) -> torch.Tensor:
     return out
 
 
+@_lib.on_stream
 def cg_direction(p, r, dinv, beta, stream=None) -> None:
     """Row i: p = dinv r + beta_i p."""
     _rows(p), _rows(r)
@@ -164,6 +168,7 @@ def cg_direction(p, r, dinv, beta, stream=None) -> None:
               p.stride(0), _lib.stream_handle(stream))
 
 
+@_lib.on_stream
 def cg_start(b, isw, q, dinv, w, r, p, stream=None) -> torch.Tensor:
     """First residual of the row-wise CG from x0 = dinv isw b (q = A x0 given): row i:
     r = isw b - q, p = dinv r; returns the FP64 vector sum w r_i p_i."""
@@ -186,6 +191,7 @@ def cg_start(b, isw, q, dinv, w, r, p, stream=None) -> torch.Tensor:
     return out
 
 
+@_lib.on_stream
 def cg_finish(x, p, sw, alpha, y, stream=None) -> None:
     """Last CG update mapped back: row i: y = sw (x + alpha_i p); x is not written."""
     _rows(x), _rows(p), _rows(y)

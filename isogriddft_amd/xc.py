@@ -60,6 +60,7 @@ def validate_density_field(rho, name="rho"):
     return np.maximum(values, 0.0)
 
 
+@_lib.on_stream
 def lsda_device(rho_up: torch.Tensor, rho_down: torch.Tensor, stream=None, functional=None):
     """(eps_xc, v_up, v_down) FP64 on the device; raises ValueError on invalid densities."""
     entry = _ENTRY[normalize_functional(functional)]
