@@ -544,15 +544,15 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
             Psi = _mm(Ct, Z)
             _mm(Ct, HZ, out=R)
             th = theta[:k].numpy()
-            R.sub_(torch.from_numpy(th).to(dev, dt)[:, None] * Psi)
+            R.addcmul_(Psi, torch.from_numpy(-th).to(dev, dt)[:, None])
             refresh = False
         else:
             Gzz = None
             P = None
             th = theta_all[:k]
             Psi = Z[:k]
-            torch.mul(Psi, torch.from_numpy(th).to(dev, dt)[:, None], out=R)
-            torch.sub(HZ[:k], R, out=R)
+            # R = HZ - theta Z in one pass
+            torch.addcmul(HZ[:k], Psi, torch.from_numpy(-th).to(dev, dt)[:, None], out=R)
         rn = np.sqrt(np.clip(np.diag(gemm_nt(R, R).cpu().numpy()), 0.0, None))
         res.residual_history.append(rn)
         res.ritz_value_history.append(th)
@@ -618,7 +618,12 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
         # project the residuals out of the basis and whiten them
         GRZ = gemm_nt(Ra, Z)
         mark("gram")
-        Ra.sub_(_mm(GRZ.to(dt), Z))
+        # R_a <- R_a - (R_a Z^T) Z as one rotation [-R_a Z^T, I] [Z; R_a] into the rows of R_a
+        # (the kernel reads all of its source columns before writing them)
+        Qp = torch.cat([-GRZ, torch.eye(ka, dtype=GRZ.dtype, device=GRZ.device)], 1).to(dt)
+        if not (Ra.is_cuda and _rotate_kernel(Qp, BUF[:m + ka], Ra)):
+            Ra.sub_(_mm(GRZ.to(dt), Z))
+        del Qp
         GR = gemm_nt(Ra, Ra).cpu()
         mark("gram")
         # |R_pre|^2 = |R|^2 + |R Z^T|^2 (Z orthonormal).  Rows that lost more than 99 % of
@@ -648,7 +653,8 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
             GR[dead, dead] = 1.0
         Linv_r = _chol_inv(GR).to(dev)
         mark("eigh")
-        Ra.copy_(_mm(Linv_r.to(dt), Ra))
+        if not (Ra.is_cuda and _rotate_kernel(Linv_r.to(dt), Ra, Ra)):  # in place
+            Ra.copy_(_mm(Linv_r.to(dt), Ra))
         apply_s(Ra, HRa)
         G23 = gemm_nt(BUF[:m + ka], BUF[m:m + 2 * ka]).cpu()  # [Z; R_a] [R_a; HR_a]^T: (m + ka, 2ka)
         # R_a HZ^T: (ka, m); equal to (Z HR_a^T)^T up to rounding when S is symmetric
