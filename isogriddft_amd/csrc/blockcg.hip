@@ -20,6 +20,18 @@ namespace {
 
 constexpr int CG_MAX_CHUNKS = 256;
 
+// Workgroup -> (row, column chunk) of a k-row, nch-chunk launch on a 1-D grid of
+// 8 k ceil(nch / 8) workgroups.  Workgroup b runs on XCD b % 8, and each XCD takes its
+// chunks (xcd, xcd + 8, ...) for every row in turn, so the per-point fields a chunk reads
+// for all rows (dinv, w, isw, sw) come into that XCD's L2 once instead of once per row.
+__device__ __forceinline__ bool row_chunk(int k, int nch, int& row, int& cx) {
+  const int b = blockIdx.x;
+  const int r = b >> 3;
+  row = r % k;
+  cx = (r / k) * 8 + (b & 7);
+  return cx < nch;
+}
+
 // 16-byte vectors: float4 / double2.  The vector kernels run when N, ld and the chunk are
 // multiples of VW and every base pointer is 16-byte aligned (the host checks), otherwise
 // the scalar ones.
@@ -47,12 +59,14 @@ __device__ __forceinline__ double vsum_prod(const typename Vec16<T>::t& a, const
 template <typename T>
 __global__ __launch_bounds__(IG_THREADS) void row_dots_vec(const T* __restrict__ a, const T* __restrict__ b,
                                                            const T* __restrict__ w, long long N, long long ld,
-                                                           long long chunk, double* __restrict__ partial) {
+                                                           long long chunk, int k, int nch,
+                                                           double* __restrict__ partial) {
   using VT = typename Vec16<T>::t;
   constexpr int VW = Vec16<T>::W;
   __shared__ double sm[4];
-  const int row = blockIdx.y;
-  const long long beg = (long long)blockIdx.x * chunk;
+  int row, cx;
+  if (!row_chunk(k, nch, row, cx)) return;
+  const long long beg = (long long)cx * chunk;
   const long long end = min(beg + chunk, N);
   const T* ar = a + row * ld;
   const T* br = b + row * ld;
@@ -70,7 +84,7 @@ __global__ __launch_bounds__(IG_THREADS) void row_dots_vec(const T* __restrict__
   }
   double v[1] = {s};
   ig_block_sum<1>(v, sm);
-  if (threadIdx.x == 0) partial[(long long)row * gridDim.x + blockIdx.x] = v[0];
+  if (threadIdx.x == 0) partial[(long long)row * nch + cx] = v[0];
 }
 
 template <typename T>
@@ -78,13 +92,14 @@ __global__ __launch_bounds__(IG_THREADS) void cg_update_vec(T* __restrict__ x, T
                                                             const T* __restrict__ p, const T* __restrict__ q,
                                                             const T* __restrict__ dinv, const T* __restrict__ w,
                                                             const double* __restrict__ alpha, long long N,
-                                                            long long ld, long long chunk,
+                                                            long long ld, long long chunk, int k, int nch,
                                                             double* __restrict__ partial) {
   using VT = typename Vec16<T>::t;
   constexpr int VW = Vec16<T>::W;
   __shared__ double sm[4];
-  const int row = blockIdx.y;
-  const long long beg = (long long)blockIdx.x * chunk;
+  int row, cx;
+  if (!row_chunk(k, nch, row, cx)) return;
+  const long long beg = (long long)cx * chunk;
   const long long end = min(beg + chunk, N);
   const T a = (T)alpha[row];
   T* xr = x + row * ld;
@@ -112,18 +127,19 @@ __global__ __launch_bounds__(IG_THREADS) void cg_update_vec(T* __restrict__ x, T
   }
   double v[1] = {s};
   ig_block_sum<1>(v, sm);
-  if (threadIdx.x == 0) partial[(long long)row * gridDim.x + blockIdx.x] = v[0];
+  if (threadIdx.x == 0) partial[(long long)row * nch + cx] = v[0];
 }
 
 template <typename T>
 __global__ __launch_bounds__(IG_THREADS) void cg_direction_vec(T* __restrict__ p, const T* __restrict__ r,
                                                                const T* __restrict__ dinv,
                                                                const double* __restrict__ beta, long long N,
-                                                               long long ld, long long chunk) {
+                                                               long long ld, long long chunk, int k, int nch) {
   using VT = typename Vec16<T>::t;
   constexpr int VW = Vec16<T>::W;
-  const int row = blockIdx.y;
-  const long long beg = (long long)blockIdx.x * chunk;
+  int row, cx;
+  if (!row_chunk(k, nch, row, cx)) return;
+  const long long beg = (long long)cx * chunk;
   const long long end = min(beg + chunk, N);
   const T b = (T)beta[row];
   T* pr = p + row * ld;
@@ -141,10 +157,12 @@ __global__ __launch_bounds__(IG_THREADS) void cg_direction_vec(T* __restrict__ p
 template <typename T>
 __global__ __launch_bounds__(IG_THREADS) void row_dots_kernel(const T* __restrict__ a, const T* __restrict__ b,
                                                               const T* __restrict__ w, long long N, long long ld,
-                                                              long long chunk, double* __restrict__ partial) {
+                                                              long long chunk, int k, int nch,
+                                                              double* __restrict__ partial) {
   __shared__ double sm[4];
-  const int row = blockIdx.y;
-  const long long beg = (long long)blockIdx.x * chunk;
+  int row, cx;
+  if (!row_chunk(k, nch, row, cx)) return;
+  const long long beg = (long long)cx * chunk;
   const long long end = min(beg + chunk, N);
   const T* ar = a + row * ld;
   const T* br = b + row * ld;
@@ -155,7 +173,7 @@ __global__ __launch_bounds__(IG_THREADS) void row_dots_kernel(const T* __restric
   }
   double v[1] = {s};
   ig_block_sum<1>(v, sm);
-  if (threadIdx.x == 0) partial[(long long)row * gridDim.x + blockIdx.x] = v[0];
+  if (threadIdx.x == 0) partial[(long long)row * nch + cx] = v[0];
 }
 
 template <typename T>
@@ -163,11 +181,12 @@ __global__ __launch_bounds__(IG_THREADS) void cg_update_kernel(T* __restrict__ x
                                                                const T* __restrict__ p, const T* __restrict__ q,
                                                                const T* __restrict__ dinv, const T* __restrict__ w,
                                                                const double* __restrict__ alpha, long long N,
-                                                               long long ld, long long chunk,
+                                                               long long ld, long long chunk, int k, int nch,
                                                                double* __restrict__ partial) {
   __shared__ double sm[4];
-  const int row = blockIdx.y;
-  const long long beg = (long long)blockIdx.x * chunk;
+  int row, cx;
+  if (!row_chunk(k, nch, row, cx)) return;
+  const long long beg = (long long)cx * chunk;
   const long long end = min(beg + chunk, N);
   const T a = (T)alpha[row];
   T* xr = x + row * ld;
@@ -184,16 +203,17 @@ __global__ __launch_bounds__(IG_THREADS) void cg_update_kernel(T* __restrict__ x
   }
   double v[1] = {s};
   ig_block_sum<1>(v, sm);
-  if (threadIdx.x == 0) partial[(long long)row * gridDim.x + blockIdx.x] = v[0];
+  if (threadIdx.x == 0) partial[(long long)row * nch + cx] = v[0];
 }
 
 template <typename T>
 __global__ __launch_bounds__(IG_THREADS) void cg_direction_kernel(T* __restrict__ p, const T* __restrict__ r,
                                                                   const T* __restrict__ dinv,
                                                                   const double* __restrict__ beta, long long N,
-                                                                  long long ld, long long chunk) {
-  const int row = blockIdx.y;
-  const long long beg = (long long)blockIdx.x * chunk;
+                                                                  long long ld, long long chunk, int k, int nch) {
+  int row, cx;
+  if (!row_chunk(k, nch, row, cx)) return;
+  const long long beg = (long long)cx * chunk;
   const long long end = min(beg + chunk, N);
   const T b = (T)beta[row];
   T* pr = p + row * ld;
@@ -206,13 +226,14 @@ __global__ __launch_bounds__(IG_THREADS) void cg_start_vec(const T* __restrict__
                                                            const T* __restrict__ isw, const T* __restrict__ q,
                                                            const T* __restrict__ dinv, const T* __restrict__ w,
                                                            T* __restrict__ r, T* __restrict__ p, long long N,
-                                                           long long ld, long long chunk,
+                                                           long long ld, long long chunk, int k, int nch,
                                                            double* __restrict__ partial) {
   using VT = typename Vec16<T>::t;
   constexpr int VW = Vec16<T>::W;
   __shared__ double sm[4];
-  const int row = blockIdx.y;
-  const long long beg = (long long)blockIdx.x * chunk;
+  int row, cx;
+  if (!row_chunk(k, nch, row, cx)) return;
+  const long long beg = (long long)cx * chunk;
   const long long end = min(beg + chunk, N);
   const T* br = b + row * ldb;
   const T* qr = q + row * ld;
@@ -239,7 +260,7 @@ __global__ __launch_bounds__(IG_THREADS) void cg_start_vec(const T* __restrict__
   }
   double v[1] = {s};
   ig_block_sum<1>(v, sm);
-  if (threadIdx.x == 0) partial[(long long)row * gridDim.x + blockIdx.x] = v[0];
+  if (threadIdx.x == 0) partial[(long long)row * nch + cx] = v[0];
 }
 
 template <typename T>
@@ -247,11 +268,12 @@ __global__ __launch_bounds__(IG_THREADS) void cg_start_kernel(const T* __restric
                                                               const T* __restrict__ isw, const T* __restrict__ q,
                                                               const T* __restrict__ dinv, const T* __restrict__ w,
                                                               T* __restrict__ r, T* __restrict__ p, long long N,
-                                                              long long ld, long long chunk,
+                                                              long long ld, long long chunk, int k, int nch,
                                                               double* __restrict__ partial) {
   __shared__ double sm[4];
-  const int row = blockIdx.y;
-  const long long beg = (long long)blockIdx.x * chunk;
+  int row, cx;
+  if (!row_chunk(k, nch, row, cx)) return;
+  const long long beg = (long long)cx * chunk;
   const long long end = min(beg + chunk, N);
   const T* br = b + row * ldb;
   const T* qr = q + row * ld;
@@ -268,7 +290,7 @@ __global__ __launch_bounds__(IG_THREADS) void cg_start_kernel(const T* __restric
   }
   double v[1] = {s};
   ig_block_sum<1>(v, sm);
-  if (threadIdx.x == 0) partial[(long long)row * gridDim.x + blockIdx.x] = v[0];
+  if (threadIdx.x == 0) partial[(long long)row * nch + cx] = v[0];
 }
 
 template <typename T>
@@ -276,11 +298,12 @@ __global__ __launch_bounds__(IG_THREADS) void cg_finish_vec(const T* __restrict_
                                                             const T* __restrict__ sw,
                                                             const double* __restrict__ alpha, T* __restrict__ y,
                                                             long long ldy, long long N, long long ld,
-                                                            long long chunk) {
+                                                            long long chunk, int k, int nch) {
   using VT = typename Vec16<T>::t;
   constexpr int VW = Vec16<T>::W;
-  const int row = blockIdx.y;
-  const long long beg = (long long)blockIdx.x * chunk;
+  int row, cx;
+  if (!row_chunk(k, nch, row, cx)) return;
+  const long long beg = (long long)cx * chunk;
   const long long end = min(beg + chunk, N);
   const T a = (T)alpha[row];
   const T* xr = x + row * ld;
@@ -302,9 +325,10 @@ __global__ __launch_bounds__(IG_THREADS) void cg_finish_kernel(const T* __restri
                                                                const T* __restrict__ sw,
                                                                const double* __restrict__ alpha, T* __restrict__ y,
                                                                long long ldy, long long N, long long ld,
-                                                               long long chunk) {
-  const int row = blockIdx.y;
-  const long long beg = (long long)blockIdx.x * chunk;
+                                                               long long chunk, int k, int nch) {
+  int row, cx;
+  if (!row_chunk(k, nch, row, cx)) return;
+  const long long beg = (long long)cx * chunk;
   const long long end = min(beg + chunk, N);
   const T a = (T)alpha[row];
   const T* xr = x + row * ld;
@@ -345,6 +369,7 @@ RowPlan row_plan(int k, long long N, int align) {
   const long long cap = (N + IG_THREADS - 1) / IG_THREADS;  // at least one element per thread
   if (nch > cap) nch = cap;
   if (nch > CG_MAX_CHUNKS) nch = CG_MAX_CHUNKS;
+  if (nch >= 8) nch -= nch % 8;  // whole groups of 8 chunks: one per XCD, none idle
   if (nch < 1) nch = 1;
   RowPlan g;
   g.chunk = (N + nch - 1) / nch;
@@ -352,6 +377,8 @@ RowPlan row_plan(int k, long long N, int align) {
   g.nch = (int)((N + g.chunk - 1) / g.chunk);
   return g;
 }
+
+inline unsigned cg_blocks(int k, int nch) { return (unsigned)(8LL * k * ((nch + 7) / 8)); }
 
 inline bool al16(const void* q) { return q == nullptr || (reinterpret_cast<uintptr_t>(q) & 15u) == 0; }
 
@@ -370,10 +397,13 @@ int row_dots(const T* a, const T* b, const T* w, int k, long long N, long long l
   if (!a || !b || !work || !out || k <= 0 || k > 65535 || N <= 0 || ld < N) return IG_EINVAL;
   const bool vec = use_vec<T>(N, ld, {a, b, w});
   const RowPlan g = row_plan(k, N, vec ? Vec16<T>::W : 1);
+  const dim3 grid(cg_blocks(k, g.nch));
   if (vec)
-    hipLaunchKernelGGL(row_dots_vec<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, a, b, w, N, ld, g.chunk, work);
+    hipLaunchKernelGGL(row_dots_vec<T>, grid, dim3(IG_THREADS), 0, s, a, b, w, N, ld, g.chunk,
+                       k, g.nch, work);
   else
-    hipLaunchKernelGGL(row_dots_kernel<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, a, b, w, N, ld, g.chunk, work);
+    hipLaunchKernelGGL(row_dots_kernel<T>, grid, dim3(IG_THREADS), 0, s, a, b, w, N, ld,
+                       g.chunk, k, g.nch, work);
   hipLaunchKernelGGL(rows_finish, dim3((k + 255) / 256), dim3(256), 0, s, work, g.nch, k, out);
   IG_CHECK_LAUNCH();
   return IG_OK;
@@ -386,12 +416,13 @@ int cg_update(T* x, T* r, const T* p, const T* q, const T* dinv, const T* w, con
     return IG_EINVAL;
   const bool vec = use_vec<T>(N, ld, {x, r, p, q, dinv, w});
   const RowPlan g = row_plan(k, N, vec ? Vec16<T>::W : 1);
+  const dim3 grid(cg_blocks(k, g.nch));
   if (vec)
-    hipLaunchKernelGGL(cg_update_vec<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, x, r, p, q, dinv, w, alpha, N, ld,
-                       g.chunk, work);
+    hipLaunchKernelGGL(cg_update_vec<T>, grid, dim3(IG_THREADS), 0, s, x, r, p, q, dinv, w, alpha, N, ld,
+                       g.chunk, k, g.nch, work);
   else
-    hipLaunchKernelGGL(cg_update_kernel<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, x, r, p, q, dinv, w, alpha, N,
-                       ld, g.chunk, work);
+    hipLaunchKernelGGL(cg_update_kernel<T>, grid, dim3(IG_THREADS), 0, s, x, r, p, q, dinv, w, alpha, N,
+                       ld, g.chunk, k, g.nch, work);
   hipLaunchKernelGGL(rows_finish, dim3((k + 255) / 256), dim3(256), 0, s, work, g.nch, k, out);
   IG_CHECK_LAUNCH();
   return IG_OK;
@@ -403,12 +434,13 @@ int cg_direction(T* p, const T* r, const T* dinv, const double* beta, int k, lon
   if (!p || !r || !dinv || !beta || k <= 0 || k > 65535 || N <= 0 || ld < N) return IG_EINVAL;
   const bool vec = use_vec<T>(N, ld, {p, r, dinv});
   const RowPlan g = row_plan(k, N, vec ? Vec16<T>::W : 1);
+  const dim3 grid(cg_blocks(k, g.nch));
   if (vec)
-    hipLaunchKernelGGL(cg_direction_vec<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, p, r, dinv, beta, N, ld,
-                       g.chunk);
+    hipLaunchKernelGGL(cg_direction_vec<T>, grid, dim3(IG_THREADS), 0, s, p, r, dinv, beta, N, ld,
+                       g.chunk, k, g.nch);
   else
-    hipLaunchKernelGGL(cg_direction_kernel<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, p, r, dinv, beta, N, ld,
-                       g.chunk);
+    hipLaunchKernelGGL(cg_direction_kernel<T>, grid, dim3(IG_THREADS), 0, s, p, r, dinv, beta, N, ld,
+                       g.chunk, k, g.nch);
   IG_CHECK_LAUNCH();
   return IG_OK;
 }
@@ -421,12 +453,13 @@ int cg_start(const T* b, long long ldb, const T* isw, const T* q, const T* dinv,
     return IG_EINVAL;
   const bool vec = use_vec<T>(N, ld, {b, isw, q, dinv, w, r, p}) && ldb % Vec16<T>::W == 0;
   const RowPlan g = row_plan(k, N, vec ? Vec16<T>::W : 1);
+  const dim3 grid(cg_blocks(k, g.nch));
   if (vec)
-    hipLaunchKernelGGL(cg_start_vec<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, b, ldb, isw, q, dinv, w, r, p, N,
-                       ld, g.chunk, work);
+    hipLaunchKernelGGL(cg_start_vec<T>, grid, dim3(IG_THREADS), 0, s, b, ldb, isw, q, dinv, w, r, p, N,
+                       ld, g.chunk, k, g.nch, work);
   else
-    hipLaunchKernelGGL(cg_start_kernel<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, b, ldb, isw, q, dinv, w, r, p,
-                       N, ld, g.chunk, work);
+    hipLaunchKernelGGL(cg_start_kernel<T>, grid, dim3(IG_THREADS), 0, s, b, ldb, isw, q, dinv, w, r, p,
+                       N, ld, g.chunk, k, g.nch, work);
   hipLaunchKernelGGL(rows_finish, dim3((k + 255) / 256), dim3(256), 0, s, work, g.nch, k, out);
   IG_CHECK_LAUNCH();
   return IG_OK;
@@ -438,12 +471,13 @@ int cg_finish(const T* x, const T* p, const T* sw, const double* alpha, T* y, lo
   if (!x || !p || !sw || !alpha || !y || k <= 0 || k > 65535 || N <= 0 || ld < N || ldy < N) return IG_EINVAL;
   const bool vec = use_vec<T>(N, ld, {x, p, sw, y}) && ldy % Vec16<T>::W == 0;
   const RowPlan g = row_plan(k, N, vec ? Vec16<T>::W : 1);
+  const dim3 grid(cg_blocks(k, g.nch));
   if (vec)
-    hipLaunchKernelGGL(cg_finish_vec<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, x, p, sw, alpha, y, ldy, N, ld,
-                       g.chunk);
+    hipLaunchKernelGGL(cg_finish_vec<T>, grid, dim3(IG_THREADS), 0, s, x, p, sw, alpha, y, ldy, N, ld,
+                       g.chunk, k, g.nch);
   else
-    hipLaunchKernelGGL(cg_finish_kernel<T>, dim3(g.nch, k), dim3(IG_THREADS), 0, s, x, p, sw, alpha, y, ldy, N, ld,
-                       g.chunk);
+    hipLaunchKernelGGL(cg_finish_kernel<T>, grid, dim3(IG_THREADS), 0, s, x, p, sw, alpha, y, ldy, N, ld,
+                       g.chunk, k, g.nch);
   IG_CHECK_LAUNCH();
   return IG_OK;
 }
