@@ -303,8 +303,14 @@ def _gemm_nt_split(A, B):
 # rows per H apply panel and columns per in-place rotation panel
 APPLY_ROWS = 64
 ROTATION_PANEL = 1 << 20
-# working-subspace size m + k from which S Z is re-applied instead of rotated
+# working-subspace size m + k from which S Z is re-applied instead of rotated: with
+# projectors (the apply is a stencil pass, the projector passes and two scaling passes)
+# and on monitor grids without them (the output scaling folds into the stencil).  224^3,
+# k = 48, m = 192: without projectors re-applying is faster (eigensolve FP64 23.3 -> 21.5 s,
+# mixed 11.8 -> 10.9 s), with the benzene projectors slower (FP64 5.57 -> 5.80 s, mixed
+# 3.00 -> 3.19 s).
 HZ_APPLY_MIN = 400
+HZ_APPLY_MIN_FOLDED = 200
 # block buffers above this size release the allocator's cached segments first
 BIG_BUFFER_BYTES = 8 << 30
 
@@ -339,7 +345,8 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
 
     hz_update: how S Z follows the new basis: "rotate" (HZ' = Q [HR; HZ], 2 m (m + k) N
     flop) or "apply" (HZ' = S Z', a bandwidth-bound block apply).  None (default) applies
-    once m + k reaches HZ_APPLY_MIN, where the rotation's flops outgrow the apply's bytes.
+    once m + k reaches HZ_APPLY_MIN (HZ_APPLY_MIN_FOLDED on monitor grids without
+    projectors), where the rotation's flops outgrow the apply's bytes.
 
     lock: soft locking.  A state whose residual norm is at most LOCK_FRACTION of the
     current phase's target contributes no residual direction to the expansion: its Ritz
@@ -354,7 +361,12 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
     if hz_update not in (None, "rotate", "apply"):
         raise ValueError(f"hz_update must be `rotate` or `apply`; received `{hz_update}`.")
     m = X0.shape[0]
-    reapply = hz_update == "apply" or (hz_update is None and m + k >= HZ_APPLY_MIN)
+    # On monitor grids without projectors the W^(1/2) on the output folds into the
+    # stencil's 1/J and V factors (one pass fewer per apply), and re-applying S Z pays from
+    # a smaller subspace on
+    fold = H.dg.kind == "monitor" and H.nonlocal_plan is None
+    reapply = hz_update == "apply" or (
+        hz_update is None and m + k >= (HZ_APPLY_MIN_FOLDED if fold else HZ_APPLY_MIN))
     # S is symmetric for the zero-ghost closure and for the legacy operator under its
     # product weights (the W the callers pass); the one-sided production ends are not
     symmetric = H.dg.kind == "legacy" or H.closure == "trial_fix"
@@ -391,9 +403,6 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
                                closure=H.closure)
         kdinv64 = 1.0 / (-0.5 * H.dg.kinetic_diagonal() + KINETIC_SHIFT)
 
-    # On monitor grids without projectors the W^(1/2) on the output folds into the
-    # stencil's 1/J and V factors (one pass fewer per apply).
-    fold = H.dg.kind == "monitor" and H.nonlocal_plan is None
     folded = {}
 
     def apply_s(Zb, out):
