@@ -51,6 +51,25 @@ int ig_monitor_stencil_f32(const float* in, float* out, int nblk, long long ld, 
                            double scale, int closure, int transpose, int interior_only,
                            int xchunk, int tile, hipStream_t stream);
 
+/* The trial_fix forward operator above plus, from the same pass, the row dots
+ *   dots[m] = sum_n w[n] in[m][n] out[m][n]   (FP64, fixed summation order)
+ * of a CG step's (p, A p) products; work holds ig_monitor_stencil_dots_work_size(nblk,
+ * nx, ny, nz) doubles of partial sums.  Returns -2 where the full-z kernel does not apply
+ * (z rows wider than one wavefront: FP32 nz > 256, FP64 nz > 128; nz or ld not a multiple
+ * of the 16-byte lane width; misaligned operands): the caller then applies the operator
+ * and takes the dots separately.  Replaces: the kinetic applies of the preconditioned
+ * iteration followed by weighted inner products, ops/kinetic.py:349-358 with
+ * ops/reductions_jax.py:89-147 (the weighted inner product). */
+long long ig_monitor_stencil_dots_work_size(int nblk, int nx, int ny, int nz);
+int ig_monitor_stencil_dots_f64(const double* in, double* out, int nblk, long long ld, int nx, int ny,
+                                int nz, const double* coef, const double* invj, const double* veff,
+                                double scale, const double* w, double* work, long long work_elems,
+                                double* dots, hipStream_t stream);
+int ig_monitor_stencil_dots_f32(const float* in, float* out, int nblk, long long ld, int nx, int ny,
+                                int nz, const float* coef, const float* invj, const float* veff,
+                                double scale, const float* w, double* work, long long work_elems,
+                                double* dots, hipStream_t stream);
+
 /* Legacy separable 7-point operator:
  *   out = scale * sum_axes [a+ (f[i+1]-f[i]) - a- (f[i]-f[i-1])] + veff * in,
  * zero ghosts; coeffs = (a-_x[nx], a+_x[nx], a-_y[ny], a+_y[ny], a-_z[nz], a+_z[nz]).

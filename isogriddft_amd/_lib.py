@@ -25,6 +25,9 @@ D = C.c_double
 SIGNATURES = {
     "ig_monitor_stencil_f64": [P, P, I, L, I, I, I, P, P, P, D, I, I, I, I, I, P],
     "ig_monitor_stencil_f32": [P, P, I, L, I, I, I, P, P, P, D, I, I, I, I, I, P],
+    "ig_monitor_stencil_dots_work_size": [I, I, I, I],
+    "ig_monitor_stencil_dots_f64": [P, P, I, L, I, I, I, P, P, P, D, P, P, L, P, P],
+    "ig_monitor_stencil_dots_f32": [P, P, I, L, I, I, I, P, P, P, D, P, P, L, P, P],
     "ig_legacy_stencil_f64": [P, P, I, L, I, I, I, P, P, D, I, I, P],
     "ig_legacy_stencil_f32": [P, P, I, L, I, I, I, P, P, D, I, I, P],
     "ig_reduction_work_size": [L],
@@ -108,7 +111,8 @@ SIGNATURES = {
     "ig_stream_copy_f32": [P, P, L, P],
     "ig_stream_read_f32": [P, L, P, P],
 }
-_RESTYPE = {"ig_gram_work_size": L, "ig_knn_work_bytes": L, "ig_gemm_nt_work_size": L}
+_RESTYPE = {"ig_gram_work_size": L, "ig_knn_work_bytes": L, "ig_gemm_nt_work_size": L,
+            "ig_monitor_stencil_dots_work_size": L}
 
 _lib = None
 

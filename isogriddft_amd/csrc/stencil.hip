@@ -43,6 +43,9 @@ struct MonParams {
               // 2 y sweep in bands of eight pairs
   int ug;     // block order of the 32-point-tile kernel: 0 units of one XCD consecutive,
               // g > 0 items of g consecutive units dealt round robin to the XCDs
+  const T* dotw;     // full-z kernel with row dots: weights w of sum_n w[n] in[m][n] out[m][n]
+  double* dpart;     // its partial sums, [nblk][units]
+  long long dparts;  // doubles available at dpart
 };
 
 // Coefficient (times 2h) of row `row` at column offset `off` of the 1D derivative.
@@ -1459,7 +1462,7 @@ __device__ __forceinline__ double lane_from<double>(int addr, double v) {
   return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo);
 }
 
-template <typename T, int NB, int GD, int TY>
+template <typename T, int NB, int GD, int TY, bool DOT = false>
 __global__ __launch_bounds__((TY + 4) * 64) void monitor_zp_kernel(MonParams<T> p) {
   using VT = typename ZVec<T>::t;
   constexpr int V = ZVec<T>::V;
@@ -1539,6 +1542,7 @@ __global__ __launch_bounds__((TY + 4) * 64) void monitor_zp_kernel(MonParams<T> 
   const rsrc_t rxy = mkbuf(p.coef + 3 * N, N), rxz = mkbuf(p.coef + 4 * N, N), ryz = mkbuf(p.coef + 5 * N, N);
   const rsrc_t rij = mkbuf(p.invj, N);
   const rsrc_t rv = mkbuf(p.veff ? p.veff : p.invj, p.veff ? N : 0);
+  const rsrc_t rdw = mkbuf(DOT ? p.dotw : p.invj, DOT ? N : 0);
 
   const int tid = threadIdx.x;
   const int row = __builtin_amdgcn_readfirstlane(tid >> 6) - 2;  // uniform per wavefront
@@ -1568,6 +1572,13 @@ __global__ __launch_bounds__((TY + 4) * 64) void monitor_zp_kernel(MonParams<T> 
   VT dz[NB];      // z part of the divergence of the previous plane
   VT gc[GD][6];   // flux coefficients, set (q - sb) % GD
   VT oij, ov;     // 1/J and V of the output plane
+  VT odw;         // dot weights of the output plane (DOT)
+  // (DOT) FP64 sums of w psi out per orbital over the rows this tile owns: a clamped last
+  // y tile recomputes rows of the previous one, which count there only
+  double dacc[NB];
+  const bool dot_rows = interior && gj >= tyi * TY;
+#pragma unroll
+  for (int n = 0; n < NB; ++n) dacc[n] = 0.0;
   auto fetch_psi = [&](int qq, int slot) {
     const unsigned pb = pbytes(qq);
 #pragma unroll
@@ -1589,6 +1600,7 @@ __global__ __launch_bounds__((TY + 4) * 64) void monitor_zp_kernel(MonParams<T> 
       const unsigned o = pbytes(qq) + ooff;
       oij = zld<T>(rij, o);
       ov = zld<T>(rv, o);
+      if (DOT) odw = zld<T>(rdw, o);
     }
   };
 
@@ -1664,6 +1676,11 @@ __global__ __launch_bounds__((TY + 4) * 64) void monitor_zp_kernel(MonParams<T> 
         }
         dz[n] = dzn;
         zst<T>(rout[n], (orb_ok[n] ? ob : BAD) + ooff, v);
+        if (DOT && dot_rows && ob != BAD) {
+          // lanes past nz and rows past ny read zero weights (range-checked loads)
+#pragma unroll
+          for (int e = 0; e < V; ++e) dacc[n] += (double)odw[e] * (double)po[e] * (double)v[e];
+        }
         // ---- publish plane s + 1
         lds_vec_st<T>(s_pl[W ^ 1][n], pw, q[n][W ^ 1]);
       }
@@ -1697,33 +1714,78 @@ __global__ __launch_bounds__((TY + 4) * 64) void monitor_zp_kernel(MonParams<T> 
       __syncthreads();
     }
   }
+  if (DOT) {
+    // wavefront sums (fixed butterfly), then the rows in order through LDS (the loop's last
+    // barrier has retired every read of s_fy): one partial per orbital and (y tile, x
+    // chunk), summed in a fixed order by the finishing kernel
+#pragma unroll
+    for (int n = 0; n < NB; ++n) {
+      double a = dot_rows ? dacc[n] : 0.0;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+      dacc[n] = a;
+    }
+    double* red = reinterpret_cast<double*>(&s_fy[0][0][0]);
+    if (ln == 0) {
+#pragma unroll
+      for (int n = 0; n < NB; ++n) red[(row + 2) * NB + n] = dacc[n];
+    }
+    __syncthreads();
+    if (tid < NB) {
+      double t = 0.0;
+      for (int r = 0; r < PY; ++r) t += red[r * NB + tid];
+      const int orb = grp * NB + tid;
+      if (orb < p.nblk) p.dpart[(long long)orb * p.units + tyi + p.nty * xci] = t;
+    }
+  }
 }
 
-template <typename T, int NB, int GD, int TY>
-int launch_monitor_zp_t(const MonParams<T>& base, hipStream_t stream, bool auto_pick = false) {
+// x chunk of the full-z kernel: ~56 planes, rounded to 4 m + 2 planes, so a chunk runs its
+// planes plus two halo steps in whole groups of 4 (224 planes: 58, 58, 58, 50)
+inline int zp_chunk(int nx) {
+  const int c = nx >= 112 ? (nx + 28) / 56 : 1;
+  int xc = (nx + c - 1) / c;
+  xc = ((xc + 1) / 4) * 4 + 2;
+  return xc > nx ? nx : xc;
+}
+
+// out[m] = sum_u part[m][u] over the units in order (the row dots of the full-z kernel)
+__global__ void zp_dots_finish(const double* __restrict__ part, int units, int nblk, double* __restrict__ out) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nblk) return;
+  double t = 0.0;
+  for (int u = 0; u < units; ++u) t += part[(long long)m * units + u];
+  out[m] = t;
+}
+
+template <typename T, int NB, int GD, int TY, bool DOT = false>
+int launch_monitor_zp_t(const MonParams<T>& base, hipStream_t stream, bool auto_pick = false,
+                        double* dots = nullptr) {
   constexpr int V = ZVec<T>::V;
   MonParams<T> p = base;
   if ((long long)p.nx * p.ny * p.nz * (long long)sizeof(T) >= (1LL << 30)) return IG_EUNSUPPORTED;
   if (p.interior_only) return IG_EUNSUPPORTED;
   if (p.nz > 64 * V || p.nz % V != 0 || p.ld % V != 0) return IG_EUNSUPPORTED;
-  if (((uintptr_t)p.in | (uintptr_t)p.out | (uintptr_t)p.coef | (uintptr_t)p.invj | (uintptr_t)p.veff) & 15)
+  if (((uintptr_t)p.in | (uintptr_t)p.out | (uintptr_t)p.coef | (uintptr_t)p.invj | (uintptr_t)p.veff |
+       (uintptr_t)(DOT ? p.dotw : nullptr)) & 15)
     return IG_EUNSUPPORTED;
   p.nty = (p.ny + TY - 1) / TY;
   p.ntz = 1;
   const long long groups = (p.nblk + NB - 1) / NB;
-  if (p.xc <= 0) {
-    const int c = p.nx >= 112 ? (p.nx + 28) / 56 : 1;
-    p.xc = (p.nx + c - 1) / c;
-    p.xc = ((p.xc + 1) / 4) * 4 + 2;
-  }
+  if (p.xc <= 0) p.xc = zp_chunk(p.nx);
   if (p.xc > p.nx) p.xc = p.nx;
   p.nxc = (p.nx + p.xc - 1) / p.xc;
   p.units = p.nty * p.nxc;
+  if (DOT && (!p.dotw || !p.dpart || !dots || (long long)p.nblk * p.units > p.dparts)) return IG_EINVAL;
   const long long items = (long long)((p.nty + 1) / 2) * p.nxc;
   const long long blocks = p.order == 0 ? 8LL * groups * ((p.units + 7) / 8) : 16LL * groups * ((items + 7) / 8);
   if (blocks <= 0 || blocks > 0x7fffffffLL) return IG_EINVAL;
   if (auto_pick && blocks < 512) return IG_EUNSUPPORTED;
-  hipLaunchKernelGGL((monitor_zp_kernel<T, NB, GD, TY>), dim3((unsigned)blocks), dim3((TY + 4) * 64), 0, stream, p);
+  hipLaunchKernelGGL((monitor_zp_kernel<T, NB, GD, TY, DOT>), dim3((unsigned)blocks), dim3((TY + 4) * 64), 0, stream,
+                     p);
+  if (DOT)
+    hipLaunchKernelGGL(zp_dots_finish, dim3((p.nblk + 255) / 256), dim3(256), 0, stream, p.dpart, p.units, p.nblk,
+                       dots);
   IG_CHECK_LAUNCH();
   return IG_OK;
 }
@@ -1968,6 +2030,35 @@ int monitor_entry(const T* in, T* out, int nblk, long long ld, int nx, int ny, i
   return launch_monitor<T>(p, closure, transpose, tile, stream);
 }
 
+// H psi with the row dots out[m] = sum_n w[n] in[m][n] out[m][n] of the same pass (the
+// (p, A p) products of a CG step): the full-z kernel, 3 orbitals per workgroup, banded
+// y-sweep order; IG_EUNSUPPORTED where that kernel does not apply.
+template <typename T>
+int monitor_dots_entry(const T* in, T* out, int nblk, long long ld, int nx, int ny, int nz, const T* coef,
+                       const T* invj, const T* veff, double scale, const T* w, double* work, long long work_elems,
+                       double* dots, hipStream_t stream) {
+  if (!in || !out || !coef || !invj || !w || !work || !dots || nblk <= 0 || nx < 3 || ny < 3 || nz < 3)
+    return IG_EINVAL;
+  if (ld < (long long)nx * ny * nz || work_elems <= 0) return IG_EINVAL;
+  MonParams<T> p{};
+  p.in = in;
+  p.out = out;
+  p.coef = coef;
+  p.invj = invj;
+  p.veff = veff;
+  p.ld = ld;
+  p.nblk = nblk;
+  p.nx = nx;
+  p.ny = ny;
+  p.nz = nz;
+  p.scale = T(scale);
+  p.order = 2;
+  p.dotw = w;
+  p.dpart = work;
+  p.dparts = work_elems;
+  return launch_monitor_zp_t<T, 3, 1, 8, true>(p, stream, false, dots);
+}
+
 template <typename T>
 int legacy_entry(const T* in, T* out, int nblk, long long ld, int nx, int ny, int nz,
                  const T* coeffs, const T* veff, double scale, int transpose, int interior_only,
@@ -2013,6 +2104,28 @@ int ig_monitor_stencil_f32(const float* in, float* out, int nblk, long long ld, 
                            int xchunk, int tile, hipStream_t stream) {
   return monitor_entry<float>(in, out, nblk, ld, nx, ny, nz, coef, invj, veff, scale,
                               closure, transpose, interior_only, xchunk, tile, stream);
+}
+
+long long ig_monitor_stencil_dots_work_size(int nblk, int nx, int ny, int nz) {
+  if (nblk <= 0 || nx < 3 || ny < 3 || nz < 3) return -1;
+  const int xc = zp_chunk(nx);
+  return (long long)nblk * ((ny + 7) / 8) * ((nx + xc - 1) / xc);
+}
+
+int ig_monitor_stencil_dots_f64(const double* in, double* out, int nblk, long long ld, int nx, int ny, int nz,
+                                const double* coef, const double* invj, const double* veff, double scale,
+                                const double* w, double* work, long long work_elems, double* dots,
+                                hipStream_t stream) {
+  return monitor_dots_entry<double>(in, out, nblk, ld, nx, ny, nz, coef, invj, veff, scale, w, work, work_elems,
+                                    dots, stream);
+}
+
+int ig_monitor_stencil_dots_f32(const float* in, float* out, int nblk, long long ld, int nx, int ny, int nz,
+                                const float* coef, const float* invj, const float* veff, double scale,
+                                const float* w, double* work, long long work_elems, double* dots,
+                                hipStream_t stream) {
+  return monitor_dots_entry<float>(in, out, nblk, ld, nx, ny, nz, coef, invj, veff, scale, w, work, work_elems,
+                                   dots, stream);
 }
 
 int ig_legacy_stencil_f64(const double* in, double* out, int nblk, long long ld, int nx, int ny,

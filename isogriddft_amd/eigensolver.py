@@ -236,6 +236,11 @@ def _chol_inv(G):
 PRECONDITIONER_FLOOR = 1e-2
 # kinetic preconditioner: CG steps per residual block and the SPD shift c (Ha)
 KINETIC_CG_STEPS = 8
+# kinetic CG: take the (p, A p) row dots from the stencil pass itself (FP32 blocks whose
+# geometry outgrows the Infinity Cache, where the fused full-z kernel is the stencil the
+# launcher would pick anyway)
+FUSE_CG_DOTS = True
+FUSE_CG_MIN_GEOMETRY_BYTES = 128 << 20
 KINETIC_SHIFT = 1.0
 # residual rows whose (twice) projection out of the basis keeps less than DEPENDENT_REL *
 # eps of their norm are treated as lying in the basis span
@@ -441,6 +446,7 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
             cg_bufs[dtb] = (torch.empty((4, k, N), dtype=dtb, device=dev), dinv_t, w64.to(dtb), isw_t,
                             isw_t * dinv_t, sw64.to(dtb))
         buf, dinv, wd, iswd, isd, swd = cg_bufs[dtb]
+        fuse_dots = FUSE_CG_DOTS and dtb == torch.float32 and N * 4 * 8 > FUSE_CG_MIN_GEOMETRY_BYTES
         kb = Rb.shape[0]
         x, r, pv, q = buf[0, :kb], buf[1, :kb], buf[2, :kb], buf[3, :kb]
         mark("rotation")
@@ -448,8 +454,12 @@ def block_subspace_iteration(H, w, X0, k, tol, max_iter, *, precision="fp64", fp
         kop.apply(x.view((kb,) + shape), q.view((kb,) + shape))
         rz = cg_start(Rb, iswd, q, dinv, wd, r, pv)     # r = W^(-1/2) R - A x0, p = D^-1 r
         for step in range(KINETIC_CG_STEPS):
-            kop.apply(pv.view((kb,) + shape), q.view((kb,) + shape))
-            pq = row_dots(pv, q, wd)
+            # q = A p with the (p, q)_W row dots from the same stencil pass where the fused
+            # kernel takes the block
+            pq = kop.apply_dots(pv.view((kb,) + shape), q.view((kb,) + shape), wd) if fuse_dots else None
+            if pq is None:
+                kop.apply(pv.view((kb,) + shape), q.view((kb,) + shape))
+                pq = row_dots(pv, q, wd)
             alpha = torch.where(pq > 0, rz / torch.where(pq > 0, pq, 1.0), 0.0)
             if step == KINETIC_CG_STEPS - 1:
                 cg_finish(x, pv, swd, alpha, Rb)        # R = W^(1/2) (x + alpha p)

@@ -24,7 +24,7 @@ import torch
 
 from . import _lib, configs
 from .device import DeviceGrid, device_grid
-from .kinetic import apply_operator
+from .kinetic import apply_operator, apply_operator_dots
 
 H2_BENCHMARK_CASE = configs.H2
 _SPIN = ("up", "down")
@@ -68,6 +68,15 @@ class BlockHamiltonian:
         return out
 
     __call__ = apply
+
+    def apply_dots(self, X: torch.Tensor, out: torch.Tensor, w: torch.Tensor) -> torch.Tensor | None:
+        """out = H X and, from the same pass, the FP64 row dots sum_n w[n] X[m][n] out[m][n];
+        None (nothing launched) where the fused kernel does not take the block or the
+        Hamiltonian carries projectors (then apply and take the dots separately)."""
+        if self.nonlocal_plan is not None:
+            return None
+        v = self.veff[X.dtype].view(self.dg.shape)
+        return apply_operator_dots(self.dg, X, out, w, scale=-0.5, closure=self.closure, veff=v)
 
     def diagonal(self) -> torch.Tensor:
         """Diagonal of -1/2 Lap + V_eff (FP64, flat; the nonlocal term is left out): the

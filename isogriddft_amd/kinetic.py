@@ -107,6 +107,63 @@ def apply_operator(
     return out.reshape(x.shape)
 
 
+
+@_lib.on_stream
+def apply_operator_dots(dg: DeviceGrid, x: torch.Tensor, out: torch.Tensor, w: torch.Tensor, *, scale: float = 1.0,
+                        closure: str = "trial_fix", veff: torch.Tensor | None = None,
+                        stream=None) -> torch.Tensor | None:
+    """out = scale * Lap(x) + veff * x (trial_fix, forward, monitor grid) for a block x
+    (k, *shape) and, from the same pass, the row dots sum_n w[n] x[m][n] out[m][n] (FP64,
+    a fixed summation order): the (p, A p) products of a CG step.  Returns the dots (FP64
+    device vector of length k), or None, with nothing launched, where the fused kernel
+    does not take the block (another closure, a legacy grid, z rows wider than one
+    wavefront); the caller then applies the operator and takes the dots separately."""
+    if dg.kind != "monitor" or _closure_id(closure) != 0:
+        return None
+    if not x.is_cuda or x.dtype not in (torch.float32, torch.float64):
+        raise ValueError("apply_operator_dots takes a float32 / float64 CUDA block")
+    shape = dg.shape
+    if tuple(x.shape[-3:]) != shape:
+        raise ValueError(f"field must match the grid shape {shape}; received {tuple(x.shape[-3:])}.")
+    xb = x.reshape(-1, *shape)
+    if not xb.is_contiguous():
+        raise ValueError("apply_operator_dots takes a contiguous block")
+    nblk = xb.shape[0]
+    if not out.is_cuda or out.dtype != x.dtype or out.numel() != xb.numel() or not out.is_contiguous():
+        raise ValueError(f"out must be a contiguous CUDA tensor of {xb.numel()} {x.dtype} values.")
+    ob = out.reshape(-1, *shape)
+    if ob.data_ptr() == xb.data_ptr():
+        raise ValueError("out must not alias the input block")
+    w = w.reshape(-1)
+    if w.dtype != x.dtype or w.numel() != dg.N or not w.is_contiguous() or not w.is_cuda:
+        raise ValueError(f"w must be a contiguous CUDA field of {dg.N} {x.dtype} values.")
+    if veff is not None:
+        veff = veff.reshape(-1)
+        if veff.numel() != dg.N:
+            raise ValueError(f"veff must have {dg.N} points; received {veff.numel()}.")
+        veff = veff.to(x.dtype).contiguous()
+    from .reductions import workspace
+
+    nx, ny, nz = shape
+    lib = _lib.load()
+    nwork = int(lib.ig_monitor_stencil_dots_work_size(nblk, nx, ny, nz))
+    work = workspace(x.device, nwork, "stencil_dots")
+    dots = torch.empty(nblk, dtype=torch.float64, device=x.device)
+    dt = x.dtype
+    sfx = "f64" if dt == torch.float64 else "f32"
+    st = getattr(lib, f"ig_monitor_stencil_dots_{sfx}")(
+        _lib.ptr(xb), _lib.ptr(ob), nblk, dg.N, nx, ny, nz, _lib.ptr(dg.get("scoef", dt)), _lib.ptr(dg.invj(dt)),
+        _lib.ptr(veff), float(scale), _lib.ptr(w), _lib.ptr(work), work.numel(), _lib.ptr(dots),
+        _lib.stream_handle())
+    if st == -2:
+        return None
+    if st != 0:
+        if st == -1:
+            raise ValueError(f"ig_monitor_stencil_dots_{sfx}: invalid arguments (status {st})")
+        raise RuntimeError(f"ig_monitor_stencil_dots_{sfx}: HIP launch failed (status {st})")
+    return dots
+
+
 def _run(psi, grid_geometry, scale, closure):
     dg = device_grid(grid_geometry)
     if isinstance(psi, torch.Tensor):
